@@ -1,0 +1,381 @@
+// GRG matmul kernels for MI355X (gfx950, CDNA4, wave64) and their C ABI.
+//
+// One kernel, "row gather", runs every part of a sweep: each level of the up
+// sweep (parents pull their children), each level of the down sweep (nodes
+// pull their parents plus their selector inputs), and the write-out passes.
+// A row's value is a plain sum of gathered k-wide rows (the GRG matrix is
+// 0/1, so there are no multiplies), seeded by fused terms, and is written
+// exactly once: no atomics anywhere.
+//
+// Work mapping (light path): a wave of 64 lanes is split into groups of
+// G = kl * es lanes, one row per group. kl lanes cover a chunk of kl * cpl
+// columns with cpl-wide vector loads (16 bytes per lane for FP64 cpl=2 or FP32
+// cpl=4), and es lanes split the row's entries; the es partial sums are
+// combined with cross-lane xor shuffles. Rows whose degree exceeds a
+// threshold are skipped there and produced instead by the heavy path in the
+// same launch: one 256-thread workgroup per row, es = 256 / kl, with a
+// shuffle + LDS reduction. Grid y covers column chunks when k > kl * cpl.
+//
+// FP32 values use compensated (TwoSum) accumulation per row; FP64 values use
+// plain FP64 adds.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/grg_hip.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(const std::string& msg, int code = 1) {
+  g_last_error = msg;
+  return code;
+}
+
+constexpr int kBlock = 256;            // threads per workgroup
+constexpr int kWavesPerBlock = kBlock / 64;
+
+template <typename T>
+struct PlainAcc {
+  T s;
+  __device__ __forceinline__ void zero() { s = T(0); }
+  __device__ __forceinline__ void add(T v) { s += v; }
+  __device__ __forceinline__ void merge_xor(int off) { s += __shfl_xor(s, off, 64); }
+  __device__ __forceinline__ void merge(const PlainAcc& o) { s += o.s; }
+  __device__ __forceinline__ T value() const { return s; }
+};
+
+// Two-float accumulator: hi carries the running FP32 sum, lo the exact
+// rounding errors of every addition (Knuth TwoSum), folded in at the end.
+struct CompAcc {
+  float hi, lo;
+  __device__ __forceinline__ void zero() { hi = 0.f; lo = 0.f; }
+  __device__ __forceinline__ void add(float v) {
+    float s = hi + v;
+    float bp = s - hi;
+    float err = (hi - (s - bp)) + (v - bp);
+    hi = s;
+    lo += err;
+  }
+  __device__ __forceinline__ void merge2(float h2, float l2) {
+    float s = hi + h2;
+    float bp = s - hi;
+    float err = (hi - (s - bp)) + (h2 - bp);
+    hi = s;
+    lo += l2 + err;
+  }
+  __device__ __forceinline__ void merge_xor(int off) {
+    float h2 = __shfl_xor(hi, off, 64);
+    float l2 = __shfl_xor(lo, off, 64);
+    merge2(h2, l2);
+  }
+  __device__ __forceinline__ void merge(const CompAcc& o) { merge2(o.hi, o.lo); }
+  __device__ __forceinline__ float value() const { return hi + lo; }
+};
+
+template <typename T, bool COMP>
+struct AccOf { using type = PlainAcc<T>; };
+template <>
+struct AccOf<float, true> { using type = CompAcc; };
+
+template <typename T, int N>
+struct VecT { typedef T type __attribute__((ext_vector_type(N))); };
+
+template <typename T, int CPL>
+__device__ __forceinline__ void load_cols(const T* p, T (&v)[CPL]) {
+  if constexpr (CPL == 1) {
+    v[0] = *p;
+  } else {
+    typename VecT<T, CPL>::type w = *reinterpret_cast<const typename VecT<T, CPL>::type*>(p);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) v[c] = w[c];
+  }
+}
+
+template <typename T, int CPL>
+__device__ __forceinline__ void store_cols(T* p, const T (&v)[CPL]) {
+  if constexpr (CPL == 1) {
+    *p = v[0];
+  } else {
+    typename VecT<T, CPL>::type w;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) w[c] = v[c];
+    *reinterpret_cast<typename VecT<T, CPL>::type*>(p) = w;
+  }
+}
+
+// Sum src rows idx[j] for j = first, first + stride, ... < end into acc.
+template <typename T, int CPL, typename Acc>
+__device__ __forceinline__ void gather_range(const int32_t* __restrict__ idx, const T* __restrict__ src, int64_t ld,
+                                             int32_t first, int32_t end, int32_t stride, Acc (&acc)[CPL]) {
+  int32_t j = first;
+  for (; j + 3 * stride < end; j += 4 * stride) {
+    const int64_t i0 = idx[j], i1 = idx[j + stride], i2 = idx[j + 2 * stride], i3 = idx[j + 3 * stride];
+    T v0[CPL], v1[CPL], v2[CPL], v3[CPL];
+    load_cols<T, CPL>(src + i0 * ld, v0);
+    load_cols<T, CPL>(src + i1 * ld, v1);
+    load_cols<T, CPL>(src + i2 * ld, v2);
+    load_cols<T, CPL>(src + i3 * ld, v3);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      acc[c].add(v0[c]);
+      acc[c].add(v1[c]);
+      acc[c].add(v2[c]);
+      acc[c].add(v3[c]);
+    }
+  }
+  for (; j < end; j += stride) {
+    const int64_t i0 = idx[j];
+    T v0[CPL];
+    load_cols<T, CPL>(src + i0 * ld, v0);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) acc[c].add(v0[c]);
+  }
+}
+
+// Every CSR segment's entries j = lane_first, lane_first + stride, ... of row.
+template <typename T, int CPL, typename Acc>
+__device__ __forceinline__ void gather_segments(const GrgHipGatherArgs& a, int64_t row, int col, int32_t lane_first,
+                                                int32_t stride, Acc (&acc)[CPL]) {
+  for (int s = 0; s < a.nseg; ++s) {
+    const GrgHipSeg& sg = a.seg[s];
+    const int32_t* ptr = sg.ptr + (row - sg.ptr_base);
+    const int32_t p0 = ptr[0], p1 = ptr[1];
+    gather_range<T, CPL>(sg.idx, static_cast<const T*>(sg.src) + col, sg.ld, p0 + lane_first, p1, stride, acc);
+  }
+}
+
+// Row-wise seed / epilogue terms (added once per row).
+template <typename T, int CPL, typename Acc>
+__device__ __forceinline__ void add_row_terms(const GrgHipGatherArgs& a, int64_t row, int col, Acc (&acc)[CPL]) {
+  for (int m = 0; m < a.nmap; ++m) {
+    const GrgHipMap& mp = a.map[m];
+    if (row >= mp.row_lo && row < mp.row_hi) {
+      const int64_t r = mp.map ? (int64_t)mp.map[row - mp.row_lo] : (row - mp.row_lo);
+      T v[CPL];
+      load_cols<T, CPL>(static_cast<const T*>(mp.src) + r * mp.ld + col, v);
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) acc[c].add(v[c]);
+    }
+  }
+  if (a.row_scale || a.col_vec) {
+    const T rs = a.row_scale ? static_cast<const T*>(a.row_scale)[row - a.scale_base] : T(1);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      const T cv = a.col_vec ? static_cast<const T*>(a.col_vec)[col + c] : T(1);
+      acc[c].add(rs * cv);
+    }
+  }
+  if (a.accumulate) {
+    T v[CPL];
+    load_cols<T, CPL>(static_cast<const T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) acc[c].add(v[c]);
+  }
+}
+
+template <typename T, int CPL, typename Acc>
+__device__ __forceinline__ void store_row(const GrgHipGatherArgs& a, int64_t row, int col, const Acc (&acc)[CPL]) {
+  T v[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) v[c] = acc[c].value();
+  store_cols<T, CPL>(static_cast<T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
+}
+
+template <typename T, int CPL, bool COMP>
+__global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherArgs a, int64_t n_light_blocks,
+                                                             int g_shift, int kl_shift) {
+  using Acc = typename AccOf<T, COMP>::type;
+  __shared__ Acc partial[kWavesPerBlock][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int KL = 1 << kl_shift;
+  Acc acc[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) acc[c].zero();
+
+  if ((int64_t)blockIdx.x < n_light_blocks) {
+    // ---- light path: one row per G-lane group ----
+    const int G = 1 << g_shift;
+    const int lg = lane & (G - 1);
+    const int e = lg >> kl_shift;
+    const int ES = G >> kl_shift;
+    const int64_t row = a.row_begin + ((((int64_t)blockIdx.x * kWavesPerBlock + wave) << (6 - g_shift)) + (lane >> g_shift));
+    const int col = ((int)blockIdx.y * KL + (lg & (KL - 1))) * CPL;
+    bool live = row < a.row_end && col < a.k;
+    if (live && a.heavy_threshold >= 0) {
+      int64_t deg = 0;
+      for (int s = 0; s < a.nseg; ++s) {
+        const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
+        deg += ptr[1] - ptr[0];
+      }
+      live = deg <= a.heavy_threshold;
+    }
+    if (live) {
+      gather_segments<T, CPL>(a, row, col, e, ES, acc);
+      if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
+    }
+    for (int off = KL; off < G; off <<= 1) {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) acc[c].merge_xor(off);
+    }
+    if (live && e == 0) store_row<T, CPL>(a, row, col, acc);
+    return;
+  }
+
+  // ---- heavy path: the whole workgroup on one row ----
+  const int64_t row = a.heavy_rows[(int64_t)blockIdx.x - n_light_blocks];
+  const int cl = threadIdx.x & (KL - 1);
+  const int e = threadIdx.x >> kl_shift;
+  const int ES = kBlock >> kl_shift;
+  const int col = ((int)blockIdx.y * KL + cl) * CPL;
+  const bool live = col < a.k;
+  if (live) {
+    gather_segments<T, CPL>(a, row, col, e, ES, acc);
+    if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
+  }
+  for (int off = KL; off < 64; off <<= 1) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) acc[c].merge_xor(off);
+  }
+  // lanes 0..KL-1 of each wave hold the wave's partial for their column lane
+  if (lane < KL) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) partial[wave][lane * CPL + c] = acc[c];
+  }
+  __syncthreads();
+  if (wave == 0 && lane < KL && live) {
+    Acc tot[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      tot[c] = partial[0][lane * CPL + c];
+      for (int w = 1; w < kWavesPerBlock; ++w) tot[c].merge(partial[w][lane * CPL + c]);
+    }
+    store_row<T, CPL>(a, row, col, tot);
+  }
+}
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+int ilog2(int v) {
+  int s = 0;
+  while ((1 << s) < v) ++s;
+  return s;
+}
+
+bool misaligned(const void* p, int64_t ld, int cpl, size_t elem) {
+  if (!p) return false;
+  return (reinterpret_cast<uintptr_t>(p) % (cpl * elem)) != 0 || (ld % cpl) != 0;
+}
+
+int validate(const GrgHipGatherArgs& a) {
+  if (a.dtype != GRG_HIP_F64 && a.dtype != GRG_HIP_F32C) return fail("unknown dtype code " + std::to_string(a.dtype));
+  if (a.row_end < a.row_begin) return fail("row_end < row_begin");
+  if (a.k < 1) return fail("k must be >= 1");
+  if (a.cpl != 1 && a.cpl != 2 && a.cpl != 4) return fail("cpl must be 1, 2 or 4");
+  if (!is_pow2(a.kl) || !is_pow2(a.es) || a.kl * a.es > 64) return fail("kl and es must be powers of two with kl * es <= 64");
+  if (a.kl * a.cpl > 64) return fail("kl * cpl must be <= 64");
+  if (a.k % a.cpl) return fail("k must be a multiple of cpl");
+  if (!a.out) return fail("out is null");
+  if (a.nseg < 0 || a.nseg > GRG_HIP_MAX_SEGS) return fail("nseg out of range");
+  if (a.nmap < 0 || a.nmap > GRG_HIP_MAX_MAPS) return fail("nmap out of range");
+  const size_t elem = a.dtype == GRG_HIP_F64 ? 8 : 4;
+  if (misaligned(a.out, a.ld_out, a.cpl, elem)) return fail("out pointer / ld_out not aligned to cpl columns");
+  for (int s = 0; s < a.nseg; ++s) {
+    if (!a.seg[s].ptr || !a.seg[s].idx || !a.seg[s].src) return fail("segment " + std::to_string(s) + " has a null pointer");
+    if (misaligned(a.seg[s].src, a.seg[s].ld, a.cpl, elem)) return fail("segment src / ld not aligned to cpl columns");
+  }
+  for (int m = 0; m < a.nmap; ++m) {
+    if (!a.map[m].src) return fail("map " + std::to_string(m) + " has a null src");
+    if (misaligned(a.map[m].src, a.map[m].ld, a.cpl, elem)) return fail("map src / ld not aligned to cpl columns");
+  }
+  if (a.num_heavy < 0 || (a.num_heavy > 0 && (!a.heavy_rows || a.heavy_threshold < 0)))
+    return fail("heavy rows need heavy_rows and heavy_threshold >= 0");
+  return 0;
+}
+
+template <typename T, int CPL, bool COMP>
+void launch_typed(const GrgHipGatherArgs& a, dim3 grid, int64_t n_light, int g_shift, int kl_shift, hipStream_t s) {
+  hipLaunchKernelGGL((row_gather_kernel<T, CPL, COMP>), grid, dim3(kBlock), 0, s, a, n_light, g_shift, kl_shift);
+}
+
+int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
+  if (int rc = validate(a)) return rc;
+  const int64_t rows = a.row_end - a.row_begin;
+  const int G = a.kl * a.es;
+  const int g_shift = ilog2(G), kl_shift = ilog2(a.kl);
+  const int64_t rows_per_block = (int64_t)kWavesPerBlock * (64 / G);
+  const int64_t n_light = (rows + rows_per_block - 1) / rows_per_block;
+  const int64_t gx = n_light + a.num_heavy;
+  if (gx == 0) return 0;
+  if (gx > 0x7fffffffLL) return fail("grid too large");
+  const int chunk = a.kl * a.cpl;
+  dim3 grid((unsigned)gx, (unsigned)((a.k + chunk - 1) / chunk));
+  if (a.dtype == GRG_HIP_F64) {
+    switch (a.cpl) {
+      case 1: launch_typed<double, 1, false>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      case 2: launch_typed<double, 2, false>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      default: launch_typed<double, 4, false>(a, grid, n_light, g_shift, kl_shift, stream); break;
+    }
+  } else {
+    switch (a.cpl) {
+      case 1: launch_typed<float, 1, true>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      case 2: launch_typed<float, 2, true>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      default: launch_typed<float, 4, true>(a, grid, n_light, g_shift, kl_shift, stream); break;
+    }
+  }
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return fail(std::string("kernel launch failed: ") + hipGetErrorString(err), 2);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int grg_hip_gather(const GrgHipGatherArgs* args, void* stream) {
+  if (!args) return fail("args is null");
+  g_last_error.clear();
+  return launch_one(*args, static_cast<hipStream_t>(stream));
+}
+
+int grg_hip_gather_batch(const GrgHipGatherArgs* args, int32_t n, void* stream) {
+  if (n < 0 || (n > 0 && !args)) return fail("bad batch");
+  g_last_error.clear();
+  for (int32_t i = 0; i < n; ++i) {
+    if (int rc = launch_one(args[i], static_cast<hipStream_t>(stream))) {
+      g_last_error = "launch " + std::to_string(i) + ": " + g_last_error;
+      return rc;
+    }
+  }
+  return 0;
+}
+
+const char* grg_hip_last_error(void) { return g_last_error.c_str(); }
+
+int32_t grg_hip_abi_version(void) { return GRG_HIP_ABI_VERSION; }
+
+int grg_hip_device_count(int32_t* count) {
+  if (!count) return fail("count is null");
+  int n = 0;
+  hipError_t err = hipGetDeviceCount(&n);
+  if (err != hipSuccess) {
+    *count = 0;
+    return fail(std::string("hipGetDeviceCount: ") + hipGetErrorString(err), 2);
+  }
+  *count = n;
+  return 0;
+}
+
+int grg_hip_stream_device(void* stream, int32_t* device) {
+  if (!device) return fail("device is null");
+  int d = -1;
+  hipError_t err = hipStreamGetDevice(static_cast<hipStream_t>(stream), &d);
+  if (err != hipSuccess) return fail(std::string("hipStreamGetDevice: ") + hipGetErrorString(err), 2);
+  *device = d;
+  return 0;
+}
+
+}  // extern "C"

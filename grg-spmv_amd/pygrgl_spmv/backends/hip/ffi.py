@@ -1,0 +1,114 @@
+"""ctypes binding of ``libgrg_hip.so`` (declared in ``include/grg_hip.h``).
+
+Error convention as the reference's FFIs (``mkl/ffi.py:237-240``,
+``cusparse/ffi.py:75-78``): a non-zero status raises
+``RuntimeError("HIP <fn> failed: status <rc>: <message>")``. There is no
+fallback: if the library is missing the import of this module fails loudly.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_int32, c_int64, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
+ABI_VERSION = 1
+F64 = 0
+F32C = 1
+MAX_SEGS = 3
+MAX_MAPS = 2
+
+
+class GrgHipSeg(ctypes.Structure):
+    _fields_ = [("ptr", c_void_p), ("idx", c_void_p), ("src", c_void_p), ("ld", c_int64), ("ptr_base", c_int64)]
+
+
+class GrgHipMap(ctypes.Structure):
+    _fields_ = [("map", c_void_p), ("src", c_void_p), ("ld", c_int64), ("row_lo", c_int64), ("row_hi", c_int64)]
+
+
+class GrgHipGatherArgs(ctypes.Structure):
+    _fields_ = [
+        ("dtype", c_int32),
+        ("accumulate", c_int32),
+        ("row_begin", c_int64),
+        ("row_end", c_int64),
+        ("k", c_int32),
+        ("cpl", c_int32),
+        ("kl", c_int32),
+        ("es", c_int32),
+        ("out", c_void_p),
+        ("ld_out", c_int64),
+        ("out_base", c_int64),
+        ("nseg", c_int32),
+        ("nmap", c_int32),
+        ("seg", GrgHipSeg * MAX_SEGS),
+        ("map", GrgHipMap * MAX_MAPS),
+        ("row_scale", c_void_p),
+        ("scale_base", c_int64),
+        ("col_vec", c_void_p),
+        ("heavy_threshold", c_int64),
+        ("heavy_rows", c_void_p),
+        ("num_heavy", c_int64),
+    ]
+
+
+class HipLibrary:
+    def __init__(self, path: Path = LIB_PATH):
+        if not Path(path).exists():
+            raise RuntimeError(
+                f"HIP extension {path} is not built; run __graft_entry__.build() -- the HIP backend has no CPU fallback"
+            )
+        lib = ctypes.CDLL(str(path))
+        lib.grg_hip_gather.argtypes = [ctypes.POINTER(GrgHipGatherArgs), c_void_p]
+        lib.grg_hip_gather.restype = ctypes.c_int
+        lib.grg_hip_gather_batch.argtypes = [ctypes.POINTER(GrgHipGatherArgs), c_int32, c_void_p]
+        lib.grg_hip_gather_batch.restype = ctypes.c_int
+        lib.grg_hip_last_error.argtypes = []
+        lib.grg_hip_last_error.restype = ctypes.c_char_p
+        lib.grg_hip_abi_version.argtypes = []
+        lib.grg_hip_abi_version.restype = c_int32
+        lib.grg_hip_device_count.argtypes = [ctypes.POINTER(c_int32)]
+        lib.grg_hip_device_count.restype = ctypes.c_int
+        lib.grg_hip_stream_device.argtypes = [c_void_p, ctypes.POINTER(c_int32)]
+        lib.grg_hip_stream_device.restype = ctypes.c_int
+        version = int(lib.grg_hip_abi_version())
+        if version != ABI_VERSION:
+            raise RuntimeError(f"libgrg_hip.so ABI version {version} != expected {ABI_VERSION}; rebuild the extension")
+        self.lib = lib
+        self.path = Path(path)
+
+    def _check(self, rc: int, fn: str) -> None:
+        if int(rc) != 0:
+            msg = self.lib.grg_hip_last_error().decode("utf-8", errors="replace")
+            raise RuntimeError(f"HIP {fn} failed: status {int(rc)}: {msg}")
+
+    def gather(self, args: GrgHipGatherArgs, stream_ptr: int) -> None:
+        self._check(self.lib.grg_hip_gather(ctypes.byref(args), c_void_p(int(stream_ptr))), "grg_hip_gather")
+
+    def gather_batch(self, array, n: int, stream_ptr: int) -> None:
+        self._check(self.lib.grg_hip_gather_batch(array, c_int32(int(n)), c_void_p(int(stream_ptr))), "grg_hip_gather_batch")
+
+    def device_count(self) -> int:
+        out = c_int32(0)
+        self._check(self.lib.grg_hip_device_count(ctypes.byref(out)), "grg_hip_device_count")
+        return int(out.value)
+
+    def stream_device(self, stream_ptr: int) -> int:
+        out = c_int32(-1)
+        self._check(self.lib.grg_hip_stream_device(c_void_p(int(stream_ptr)), ctypes.byref(out)), "grg_hip_stream_device")
+        return int(out.value)
+
+
+_LIB: HipLibrary | None = None
+
+
+def hip_library() -> HipLibrary:
+    global _LIB
+    if _LIB is None:
+        _LIB = HipLibrary()
+    return _LIB
+
+
+__all__ = ["ABI_VERSION", "F32C", "F64", "GrgHipGatherArgs", "GrgHipMap", "GrgHipSeg", "HipLibrary", "LIB_PATH", "hip_library"]

@@ -1,0 +1,676 @@
+"""``HipRuntime``: runtime-owned HIP execution of the GRG matmul.
+
+Counterpart of the reference's ``TritonRuntime`` (``triton/backend.py:609-1225``)
+with the same protocol towards ``BoundGRG``: everything is allocated in
+``__enter__`` and released in ``__exit__`` (also when entering fails);
+``.grgs`` raises before entry; ``_call_scope`` rejects re-entrant calls;
+``_prepare_matmul_cuda`` returns a prepared op whose ``input`` / ``output`` /
+``miss_input`` / ``miss_output`` / ``init_vector`` / ``init_matrix`` tensors
+are transposed views of one shared arena (so one direction's output is the
+other direction's input); ``op()`` performs no allocation, host sync or
+device read-back and can be captured into a hipGraph.
+
+Execution differs from the reference by design:
+
+* the resident blocks of each destination level are merged on the device, at
+  ``__enter__``, into one CSR over global node ids -- parent-major (children
+  of each node) for the up sweep, child-major (parents of each node) for the
+  down sweep -- so each level is ONE launch that writes each row once, with
+  no atomics;
+* the seed pass (input, by-individual expansion, selector scatter, missing
+  data, init vector / matrix / XTX) is fused into the level launches as
+  extra gather terms, and the write-out (selector gather, missing output,
+  endpoint bias, by-individual reduce, emit_all_nodes permutation) is one
+  more launch;
+* a whole resident sweep is a single ``grg_hip_gather_batch`` call on the
+  runtime's root stream; streamed (non-resident) blocks are copied from
+  pinned host memory into ring slots on per-slot copy streams, each copy
+  waiting only for the previous compute that used its slot, and are then
+  accumulated into their destination level.
+"""
+
+from __future__ import annotations
+
+from contextlib import contextmanager
+from dataclasses import dataclass, field, replace
+import math
+from pathlib import Path
+
+import numpy as np
+import scipy.sparse as sp
+
+from pygrgl_spmv.backends.base import copy_struct_checked, relink_stream_dependencies
+from pygrgl_spmv.backends.hip import ffi
+from pygrgl_spmv.backends.hip.layout import HipLayout, enabled_max_k, side_io_rows
+from pygrgl_spmv.backends.types import Direction, InitMode
+from pygrgl_spmv.grg import BoundGRG, _CudaMatmulSpec
+from pygrgl_spmv.grg.artifact import _load_grg_spmv_host, iter_artifact_blocks
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+# --------------------------------------------------------------------------
+# Launch-shape selection
+# --------------------------------------------------------------------------
+
+
+def _pow2_ceil(v: int) -> int:
+    return 1 if v <= 1 else 1 << (int(v) - 1).bit_length()
+
+
+def _pow2_floor(v: float) -> int:
+    return 1 if v < 2 else 1 << (int(v).bit_length() - 1)
+
+
+def launch_shape(dtype_code: int, k: int, lds, avg_degree: float, *, es_cap: int = 16) -> tuple[int, int, int]:
+    """(cpl, kl, es): 16-byte vector columns when k and every leading dimension allow,
+    kl lanes to cover the k columns (at most 64 columns per chunk), and es lanes per row
+    so that each lane handles about two entries on average."""
+    cands = (4, 2, 1) if dtype_code == ffi.F32C else (2, 1)
+    cpl = 1
+    for c in cands:
+        if k % c == 0 and all(int(ld) % c == 0 for ld in lds):
+            cpl = c
+            break
+    kl = min(_pow2_ceil(-(-k // cpl)), 64 // cpl)
+    es = max(1, min(_pow2_floor(max(avg_degree, 1.0) / 2.0), 64 // kl, es_cap))
+    return cpl, kl, es
+
+
+# --------------------------------------------------------------------------
+# Device-side artifact state
+# --------------------------------------------------------------------------
+
+
+@dataclass
+class _StreamStep:
+    dst_level: int
+    src_level: int
+    slot: int
+    host_indices: object  # pinned torch int32
+    host_indptr: object   # pinned torch int32
+    row_lo: int
+    row_hi: int
+    src_offset: int
+    nnz: int
+    prev_in_slot: tuple[int, int] | None = None
+
+
+@dataclass
+class _HipArtifact:
+    path: Path
+    state: object
+    level_offsets: np.ndarray
+    dev: dict = field(default_factory=dict)           # name -> device tensor
+    level_avg_degree: dict = field(default_factory=dict)  # (config, level) -> float
+    heavy: dict = field(default_factory=dict)         # (config, level) -> (offset, count)
+    up_stream: list = field(default_factory=list)     # per level: list[_StreamStep]
+    down_stream: list = field(default_factory=list)
+    up_events: list = field(default_factory=list)     # per level: list[(copy_done, compute_done)]
+    down_events: list = field(default_factory=list)
+
+
+def _i32(values, label: str) -> np.ndarray:
+    arr = np.asarray(values)
+    out = np.empty(arr.shape, dtype=np.int32)
+    copy_struct_checked(out, arr.astype(np.int64, copy=False) if arr.dtype not in (np.int32, np.int64) else arr, label=label)
+    return out
+
+
+def _csr_transpose(indptr, indices, n_rows: int, n_cols: int):
+    m = sp.csr_matrix((np.ones(len(indices), dtype=np.int8), np.asarray(indices, dtype=np.int64), np.asarray(indptr, dtype=np.int64)), shape=(n_rows, n_cols))
+    t = m.T.tocsr()
+    t.sort_indices()
+    return np.asarray(t.indptr, dtype=np.int64), np.asarray(t.indices, dtype=np.int64)
+
+
+class _HipPreparedMatmul:
+    """Prepared op (reference ``_TritonPreparedMatmul``, ``triton/backend.py:564-606``)."""
+
+    def __init__(self, runtime: "HipRuntime", artifact: _HipArtifact, spec: _CudaMatmulSpec) -> None:
+        self._runtime = runtime
+        self._artifact = artifact
+        self._spec = spec
+        k = int(spec.k)
+        st = artifact.state
+        self._input_internal = runtime._io0[: spec.input_cols, :k]
+        self.input = self._input_internal.T
+        if spec.emit_all_nodes:
+            self._output_internal = runtime._aux[: st.num_nodes, :k]
+        else:
+            self._output_internal = runtime._io0[: spec.output_cols, :k]
+        self.output = self._output_internal.T
+        if spec.use_miss:
+            miss_view = runtime._io1[: st.num_mutations, :k]
+            if spec.direction == Direction.DOWN:
+                self._miss_input_internal = miss_view
+                self.miss_input = miss_view.T
+            else:
+                self._miss_output_internal = miss_view
+                self.miss_output = miss_view.T
+        if spec.init_mode == InitMode.VECTOR:
+            self._init_vector_internal = runtime._io1[0, :k]
+            self.init_vector = self._init_vector_internal
+        if spec.init_mode == InitMode.MATRIX:
+            self._init_matrix_internal = runtime._io1[: st.num_nodes, :k]
+            self.init_matrix = self._init_matrix_internal.T
+        self._steps = runtime._build_steps(artifact, spec)
+
+    def __enter__(self) -> "_HipPreparedMatmul":
+        return self
+
+    def __exit__(self, exc_type, exc, tb) -> None:
+        return None
+
+    def __call__(self) -> None:
+        with self._runtime._call_scope():
+            self._call_prelocked()
+
+    def _call_prelocked(self) -> None:
+        self._runtime._execute(self._artifact, self._spec, self._steps)
+
+
+class HipRuntime:
+    """Runtime-owned HIP execution with resident or streamed sparse blocks."""
+
+    def __init__(self, layout: HipLayout) -> None:
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("HIP runtime requires a ROCm GPU (torch.cuda.is_available() is False)")
+        self._lib = ffi.hip_library()
+        self.layout = layout
+        self.device = torch.device("cuda", int(layout.device))
+        self.stream_ptr = int(layout.stream_ptr)
+        self._dtype_code = ffi.F64 if np.dtype(layout.dtype) == np.float64 else ffi.F32C
+        self._reset_owned_state()
+
+    # ---- lifecycle -------------------------------------------------------
+
+    def _reset_owned_state(self) -> None:
+        self.stream = None
+        self._caller_stream = None
+        self._root_stream = None
+        self._caller_to_root_event = None
+        self._root_to_caller_event = None
+        self._launch_event = None
+        self._slot_copy_streams = []
+        self._slot_indices = []
+        self._slot_indptr = []
+        self._state = None
+        self._io0 = None
+        self._io1 = None
+        self._aux = None
+        self._artifacts = ()
+        self._grgs = ()
+        self._entered = False
+        self._active_call = False
+
+    @property
+    def grgs(self) -> tuple:
+        if not self._entered:
+            raise RuntimeError("HipRuntime must be entered before accessing grgs")
+        return self._grgs
+
+    def _torch_dtype(self):
+        torch = _torch()
+        return torch.float64 if np.dtype(self.layout.dtype) == np.float64 else torch.float32
+
+    def _torch_caller_stream(self):
+        return self._caller_stream
+
+    def __enter__(self) -> "HipRuntime":
+        torch = _torch()
+        try:
+            lay = self.layout
+            max_k = enabled_max_k(lay.pair, lay.requirements)
+            io0_rows = max(lay.max_num_samples, lay.max_num_mutations)
+            io1_rows = side_io_rows(lay.pair, lay.requirements, max_num_mutations=lay.max_num_mutations, max_num_nodes=lay.max_num_nodes)
+            aux_rows = max(lay.max_num_nodes, lay.max_num_samples, lay.max_num_mutations, lay.max_selector_nnz)
+            tdt = self._torch_dtype()
+            with torch.cuda.device(self.device):
+                self._caller_stream = torch.cuda.get_stream_from_external(self.stream_ptr, device=self.device)
+                self.stream = self._caller_stream
+                self._root_stream = torch.cuda.Stream(device=self.device)
+                self._caller_to_root_event = torch.cuda.Event()
+                self._root_to_caller_event = torch.cuda.Event()
+                self._launch_event = torch.cuda.Event()
+                self._slot_copy_streams = [torch.cuda.Stream(device=self.device) for _ in range(lay.allocated_ring_buffer_size)]
+                for slot in lay.slot_plans:
+                    if slot.indices_dtype != np.dtype(np.int32) or slot.indptr_dtype != np.dtype(np.int32):
+                        raise NotImplementedError("HIP ring slots hold int32 structure; a block exceeds the int32 range")
+                self._slot_indices = [torch.zeros((max(1, s.indices_len),), dtype=torch.int32, device=self.device) for s in lay.slot_plans]
+                self._slot_indptr = [torch.zeros((max(1, s.indptr_len),), dtype=torch.int32, device=self.device) for s in lay.slot_plans]
+                self._state = torch.zeros((lay.max_num_nodes * max_k,), dtype=tdt, device=self.device)
+                self._io0 = torch.zeros((io0_rows, max_k), dtype=tdt, device=self.device)
+                self._io1 = None if io1_rows == 0 else torch.zeros((io1_rows, max_k), dtype=tdt, device=self.device)
+                self._aux = torch.zeros((aux_rows, max_k), dtype=tdt, device=self.device)
+            states = tuple(_load_grg_spmv_host(a.path, lay.dtype) for a in lay.artifacts)
+            arts = []
+            for ai, (al, st) in enumerate(zip(lay.artifacts, states)):
+                arts.append(self._build_artifact(ai, al, st))
+            self._artifacts = tuple(arts)
+            torch.cuda.synchronize(self.device)
+            self._grgs = tuple(BoundGRG(self, i, a.state, a.path, lay.device) for i, a in enumerate(self._artifacts))
+            self._entered = True
+            return self
+        except BaseException:
+            self._reset_owned_state()
+            raise
+
+    def __exit__(self, exc_type, exc, tb) -> None:
+        if self._entered:
+            _torch().cuda.synchronize(self.device)
+        self._reset_owned_state()
+
+    @contextmanager
+    def _call_scope(self):
+        if not self._entered:
+            raise RuntimeError("HipRuntime must be entered before matmul")
+        if self._active_call:
+            raise RuntimeError("concurrent runtime.grgs calls are not supported")
+        self._active_call = True
+        try:
+            yield
+        finally:
+            self._active_call = False
+
+    @contextmanager
+    def _caller_root_scope(self):
+        torch = _torch()
+        with torch.cuda.device(self.device):
+            with torch.cuda.stream(self._caller_stream):
+                self._caller_to_root_event.record(self._caller_stream)
+            self._root_stream.wait_event(self._caller_to_root_event)
+            try:
+                yield
+            finally:
+                with torch.cuda.stream(self._root_stream):
+                    self._root_to_caller_event.record(self._root_stream)
+                self._caller_stream.wait_event(self._root_to_caller_event)
+
+    def _prepare_matmul_cuda(self, grg: BoundGRG, spec: _CudaMatmulSpec) -> _HipPreparedMatmul:
+        if spec.direction == Direction.UP and self.layout.pair.plan_up is None:
+            raise ValueError("UP plan is not configured")
+        if spec.direction == Direction.DOWN and self.layout.pair.plan_down is None:
+            raise ValueError("DOWN plan is not configured")
+        return _HipPreparedMatmul(self, self._artifacts[int(grg._artifact_index)], spec)
+
+    # ---- artifact construction ------------------------------------------
+
+    def _to_dev(self, arr: np.ndarray, dtype=None):
+        torch = _torch()
+        a = np.ascontiguousarray(arr)
+        if a.size == 0:
+            a = np.zeros(1, dtype=a.dtype)
+        t = torch.from_numpy(a).to(device=self.device)
+        return t if dtype is None else t.to(dtype)
+
+    def _pinned(self, values: np.ndarray):
+        torch = _torch()
+        host = torch.empty((max(1, int(np.asarray(values).size)),), dtype=torch.int32, pin_memory=True)
+        if np.asarray(values).size:
+            copy_struct_checked(host.numpy()[: np.asarray(values).size], np.asarray(values), label="streamed block structure")
+        return host
+
+    def _merge_levels(self, parts_by_level, offsets: np.ndarray):
+        """Concatenate per-level CSR parts (global column ids) into one CSR over all rows.
+
+        ``parts_by_level[l]`` is a list of (indptr int64[rows_l + 1], cols int32[nnz]) device
+        tensors whose rows are the nodes of level l; parts are laid out in list order within
+        each row. Returns (ptr int32[N + 1], idx int32[max(E, 1)], degree int64[N]) on device.
+        """
+        torch = _torch()
+        dev = self.device
+        N = int(offsets[-1])
+        deg = torch.zeros(N, dtype=torch.int64, device=dev)
+        for l, parts in enumerate(parts_by_level):
+            lo, hi = int(offsets[l]), int(offsets[l + 1])
+            for ptr, _ in parts:
+                deg[lo:hi] += ptr[1:] - ptr[:-1]
+        ptr_all = torch.zeros(N + 1, dtype=torch.int64, device=dev)
+        torch.cumsum(deg, 0, out=ptr_all[1:])
+        total = int(ptr_all[-1].item())
+        if total > np.iinfo(np.int32).max:
+            raise NotImplementedError("more than 2^31 - 1 resident entries in one sweep orientation")
+        idx = torch.zeros(max(total, 1), dtype=torch.int32, device=dev)
+        for l, parts in enumerate(parts_by_level):
+            lo, hi = int(offsets[l]), int(offsets[l + 1])
+            cursor = ptr_all[lo:hi].clone()
+            rows = hi - lo
+            for ptr, cols in parts:
+                d = ptr[1:] - ptr[:-1]
+                nnz = int(cols.numel())
+                if nnz == 0:
+                    continue
+                row_of = torch.repeat_interleave(torch.arange(rows, device=dev), d)
+                pos = cursor[row_of] + (torch.arange(nnz, device=dev) - ptr[:-1][row_of])
+                idx[pos] = cols
+                cursor += d
+        return ptr_all.to(torch.int32), idx, deg
+
+    def _build_artifact(self, ai: int, al, st) -> _HipArtifact:
+        torch = _torch()
+        lay = self.layout
+        dev = self.device
+        w_dtype = self._torch_dtype()
+        off = np.asarray(st.level_offsets, dtype=np.int64)
+        L = off.size - 1
+        S, M, N, I = st.num_samples, st.num_mutations, st.num_nodes, st.num_individuals
+        art = _HipArtifact(path=al.path, state=st, level_offsets=off)
+        D = art.dev
+        up_plan = {(b.dst_level, b.src_level): b for b in al.blocks_up}
+        dn_plan = {(b.dst_level, b.src_level): b for b in al.blocks_down}
+        up_parts = [[] for _ in range(L)]   # by parent level, ascending source level
+        dn_parts = {l: [] for l in range(L)}  # by child level: (parent level, part)
+        up_stream = [[] for _ in range(L)]
+        dn_stream = [[] for _ in range(L)]
+        with torch.cuda.device(dev):
+            for blk in iter_artifact_blocks(al.path):
+                if blk.nnz == 0:
+                    continue
+                d, s = blk.dst_level, blk.src_level
+                ptr_h = np.asarray(blk.indptr, dtype=np.int64)
+                idx_h = np.asarray(blk.indices, dtype=np.int64)
+                bp = up_plan.get((d, s))
+                if bp is not None:
+                    if bp.resident:
+                        ptr_d = self._to_dev(ptr_h)
+                        cols_d = (self._to_dev(idx_h) + int(off[s])).to(torch.int32)
+                        up_parts[d].append((ptr_d, cols_d))
+                    else:
+                        up_stream[d].append(_StreamStep(
+                            dst_level=d, src_level=s, slot=int(bp.slot), host_indices=self._pinned(idx_h),
+                            host_indptr=self._pinned(ptr_h), row_lo=int(off[d]), row_hi=int(off[d + 1]),
+                            src_offset=int(off[s]), nnz=int(idx_h.size)))
+                bp = dn_plan.get((d, s))
+                if bp is not None:
+                    if bp.resident:
+                        ptr_d = self._to_dev(ptr_h)
+                        cols_d = self._to_dev(idx_h)
+                        rows_d = int(off[d + 1] - off[d])
+                        row_of = torch.repeat_interleave(torch.arange(rows_d, device=dev), ptr_d[1:] - ptr_d[:-1])
+                        keys, order = torch.sort(cols_d, stable=True)
+                        parents = (row_of[order] + int(off[d])).to(torch.int32)
+                        counts = torch.bincount(keys, minlength=int(off[s + 1] - off[s]))
+                        tptr = torch.zeros(counts.numel() + 1, dtype=torch.int64, device=dev)
+                        torch.cumsum(counts, 0, out=tptr[1:])
+                        dn_parts[s].append((d, (tptr, parents)))
+                    else:
+                        tptr_h, tidx_h = _csr_transpose(ptr_h, idx_h, int(off[d + 1] - off[d]), int(off[s + 1] - off[s]))
+                        dn_stream[s].append(_StreamStep(
+                            dst_level=s, src_level=d, slot=int(bp.slot), host_indices=self._pinned(tidx_h),
+                            host_indptr=self._pinned(tptr_h), row_lo=int(off[s]), row_hi=int(off[s + 1]),
+                            src_offset=int(off[d]), nnz=int(tidx_h.size)))
+            if lay.pair.plan_up is not None:
+                D["up_ptr"], D["up_idx"], deg_up = self._merge_levels(up_parts, off)
+            else:
+                deg_up = None
+            if lay.pair.plan_down is not None:
+                ordered = [[part for _, part in sorted(dn_parts[l], key=lambda t: -t[0])] for l in range(L)]
+                D["dn_ptr"], D["dn_idx"], deg_dn = self._merge_levels(ordered, off)
+            else:
+                deg_dn = None
+            del up_parts, dn_parts
+            # selectors
+            sel_ptr = np.asarray(st.sel_mut.indptr, dtype=np.int64)
+            sel_idx = np.asarray(st.sel_mut.indices, dtype=np.int64)
+            miss_ptr = np.asarray(st.sel_miss.indptr, dtype=np.int64)
+            miss_idx = np.asarray(st.sel_miss.indices, dtype=np.int64)
+            req = lay.requirements
+            if lay.pair.plan_up is not None:
+                D["sel_ptr"], D["sel_idx"] = self._to_dev(_i32(sel_ptr, "sel_mut.indptr")), self._to_dev(_i32(sel_idx, "sel_mut.indices"))
+                if req.need_up_miss_output:
+                    D["miss_ptr"], D["miss_idx"] = self._to_dev(_i32(miss_ptr, "sel_miss.indptr")), self._to_dev(_i32(miss_idx, "sel_miss.indices"))
+            deg_mut_t = deg_miss_t = None
+            if lay.pair.plan_down is not None:
+                tp, ti = _csr_transpose(sel_ptr, sel_idx, M, N)
+                D["selT_ptr"], D["selT_idx"] = self._to_dev(_i32(tp, "sel_mut^T.indptr")), self._to_dev(_i32(ti, "sel_mut^T.indices"))
+                deg_mut_t = np.diff(tp)
+                if req.need_down_miss_input:
+                    tp, ti = _csr_transpose(miss_ptr, miss_idx, M, N)
+                    D["missT_ptr"], D["missT_idx"] = self._to_dev(_i32(tp, "sel_miss^T.indptr")), self._to_dev(_i32(ti, "sel_miss^T.indices"))
+                    deg_miss_t = np.diff(tp)
+            # per-node / per-sample metadata
+            node_perm = np.asarray(st.node_perm, dtype=np.int64)
+            D["node_perm"] = self._to_dev(_i32(node_perm, "node_perm"))
+            D["inv_perm"] = self._to_dev(_i32(np.asarray(st.inv_node_perm, dtype=np.int64), "inv_node_perm"))
+            s2i = np.asarray(st.sample_to_individual, dtype=np.int64)
+            D["s2i"] = self._to_dev(_i32(s2i, "sample_to_individual"))
+            order = np.argsort(s2i, kind="stable")
+            ind_ptr = np.zeros(I + 1, dtype=np.int64)
+            np.cumsum(np.bincount(s2i, minlength=I), out=ind_ptr[1:])
+            D["ind_ptr"], D["ind_idx"] = self._to_dev(_i32(ind_ptr, "individual.indptr")), self._to_dev(_i32(order, "individual.indices"))
+            npdt = np.dtype(lay.dtype)
+            if req.need_init_xtx and st.coalescence_counts is not None:
+                D["coal2"] = self._to_dev((2.0 * np.asarray(st.coalescence_counts, dtype=np.float64)).astype(npdt))
+            if req.need_init_vector:
+                if lay.pair.plan_up is not None:
+                    D["vec_up"] = self._to_dev(np.asarray(st.init_vector_up_bias, dtype=npdt))
+                if lay.pair.plan_down is not None:
+                    vd = np.asarray(st.init_vector_down_bias, dtype=np.float64)
+                    D["vec_dn"] = self._to_dev(vd.astype(npdt))
+                    D["vec_dn_ind"] = self._to_dev(np.bincount(s2i, weights=vd, minlength=I).astype(npdt))
+            if req.need_init_xtx and st.init_xtx_up_bias is not None:
+                if lay.pair.plan_up is not None:
+                    D["xtx_up"] = self._to_dev(np.asarray(st.init_xtx_up_bias, dtype=npdt))
+                if lay.pair.plan_down is not None:
+                    xd = np.asarray(st.init_xtx_down_bias, dtype=np.float64)
+                    D["xtx_dn"] = self._to_dev(xd.astype(npdt))
+                    D["xtx_dn_ind"] = self._to_dev(np.bincount(s2i, weights=xd, minlength=I).astype(npdt))
+            # heavy rows and per-level average degrees for every sweep configuration
+            H = int(lay.heavy_threshold)
+            configs = {}
+            if deg_up is not None:
+                configs["up"] = deg_up.cpu().numpy()
+            if deg_dn is not None:
+                base = deg_dn.cpu().numpy() + deg_mut_t
+                configs["down"] = base
+                if deg_miss_t is not None:
+                    configs["down_miss"] = base + deg_miss_t
+            heavy_lists = []
+            cursor = 0
+            for name, deg in configs.items():
+                for l in range(L):
+                    seg = deg[off[l] : off[l + 1]]
+                    rows = np.flatnonzero(seg > H) + off[l]
+                    art.heavy[(name, l)] = (cursor, int(rows.size))
+                    light = seg[seg <= H]
+                    art.level_avg_degree[(name, l)] = float(light.mean()) if light.size else 0.0
+                    heavy_lists.append(rows)
+                    cursor += int(rows.size)
+            cap = lay.heavy_capacity[ai] if ai < len(lay.heavy_capacity) else cursor
+            if cursor > cap:
+                raise RuntimeError(f"internal error: {cursor} heavy rows exceed the planned capacity {cap}")
+            D["heavy"] = self._to_dev(_i32(np.concatenate(heavy_lists) if heavy_lists else np.zeros(0, np.int64), "heavy rows"))
+            # streamed ops with slot-reuse links, and their events
+            art.up_stream = relink_stream_dependencies(up_stream, direction=Direction.UP)
+            art.down_stream = relink_stream_dependencies(dn_stream, direction=Direction.DOWN)
+            art.up_events = [[(torch.cuda.Event(), torch.cuda.Event()) for _ in ops] for ops in art.up_stream]
+            art.down_events = [[(torch.cuda.Event(), torch.cuda.Event()) for _ in ops] for ops in art.down_stream]
+        return art
+
+    # ---- launch construction ----------------------------------------------
+
+    def _elem(self) -> int:
+        return 8 if self._dtype_code == ffi.F64 else 4
+
+    def _args(self, *, rows, k, out, ld_out, segs=(), maps=(), row_scale=None, col_vec=None, accumulate=False,
+              heavy=None, avg_degree=1.0, out_base=0) -> ffi.GrgHipGatherArgs:
+        a = ffi.GrgHipGatherArgs()
+        a.dtype = self._dtype_code
+        a.accumulate = 1 if accumulate else 0
+        a.row_begin, a.row_end = int(rows[0]), int(rows[1])
+        a.k = int(k)
+        lds = [ld_out] + [s[3] for s in segs] + [m[2] for m in maps]
+        a.cpl, a.kl, a.es = launch_shape(self._dtype_code, int(k), lds, avg_degree)
+        a.out, a.ld_out, a.out_base = int(out), int(ld_out), int(out_base)
+        a.nseg = len(segs)
+        for i, (ptr, idx, src, ld, base) in enumerate(segs):
+            a.seg[i].ptr, a.seg[i].idx, a.seg[i].src, a.seg[i].ld, a.seg[i].ptr_base = int(ptr), int(idx), int(src), int(ld), int(base)
+        a.nmap = len(maps)
+        for i, (mp, src, ld, lo, hi) in enumerate(maps):
+            a.map[i].map, a.map[i].src, a.map[i].ld, a.map[i].row_lo, a.map[i].row_hi = int(mp or 0), int(src), int(ld), int(lo), int(hi)
+        if row_scale is not None:
+            a.row_scale, a.scale_base = int(row_scale[0]), int(row_scale[1])
+        a.col_vec = int(col_vec or 0)
+        if heavy is None:
+            a.heavy_threshold, a.heavy_rows, a.num_heavy = -1, 0, 0
+        else:
+            a.heavy_threshold, a.heavy_rows, a.num_heavy = int(heavy[0]), int(heavy[1]), int(heavy[2])
+        return a
+
+    def _build_steps(self, art: _HipArtifact, spec: _CudaMatmulSpec):
+        """The prepared sweep: a list of ("batch", array, n) and ("stream", step, args, events) items."""
+        D = art.dev
+        st = art.state
+        off = art.level_offsets
+        L = off.size - 1
+        k = int(spec.k)
+        w = self._elem()
+        V = self._state.data_ptr()
+        ldv = k
+        max_k = int(self._io0.shape[1])
+        io0 = self._io0.data_ptr()
+        io1 = self._io1.data_ptr() if self._io1 is not None else 0
+        H = int(self.layout.heavy_threshold)
+        heavy_base = D["heavy"].data_ptr()
+
+        def heavy_for(config, level):
+            o, n = art.heavy[(config, level)]
+            return (H, heavy_base + 4 * o, n)
+
+        # init terms folded into the sweep (backend_init_mode, as reference grg/__init__.py:498-502)
+        maps_init = []
+        row_scale_init = None
+        col_vec_init = None
+        if spec.backend_init_mode == InitMode.MATRIX:
+            maps_init.append((D["node_perm"].data_ptr(), io1, max_k, 0, st.num_nodes))
+        elif spec.backend_init_mode == InitMode.XTX:
+            if "coal2" not in D:
+                raise ValueError("init_mode=xtx requires GRG coalescence counts")
+            row_scale_init = (D["coal2"].data_ptr(), 0)
+        elif spec.backend_init_mode == InitMode.VECTOR:
+            col_vec_init = io1  # init vector lives in io1 row 0
+
+        items = []
+        batch: list = []
+
+        def flush():
+            if batch:
+                arr = (ffi.GrgHipGatherArgs * len(batch))(*batch)
+                items.append(("batch", arr, len(batch)))
+                batch.clear()
+
+        if spec.direction == Direction.UP:
+            levels = range(L)
+            streams, events, cfg = art.up_stream, art.up_events, "up"
+        else:
+            levels = range(L - 1, -1, -1)
+            streams, events, cfg = art.down_stream, art.down_events, ("down_miss" if spec.use_miss else "down")
+        for l in levels:
+            lo, hi = int(off[l]), int(off[l + 1])
+            segs = []
+            maps = list(maps_init)
+            if spec.direction == Direction.UP:
+                if l > 0:
+                    segs.append((D["up_ptr"].data_ptr(), D["up_idx"].data_ptr(), V, ldv, 0))
+                else:
+                    s2i = D["s2i"].data_ptr() if spec.by_individual else 0
+                    maps.insert(0, (s2i, io0, max_k, 0, st.num_samples))
+            else:
+                segs.append((D["dn_ptr"].data_ptr(), D["dn_idx"].data_ptr(), V, ldv, 0))
+                segs.append((D["selT_ptr"].data_ptr(), D["selT_idx"].data_ptr(), io0, max_k, 0))
+                if spec.use_miss:
+                    segs.append((D["missT_ptr"].data_ptr(), D["missT_idx"].data_ptr(), io1, max_k, 0))
+            heavy = heavy_for(cfg, l) if segs else None
+            batch.append(self._args(rows=(lo, hi), k=k, out=V, ld_out=ldv, segs=segs, maps=maps,
+                                    row_scale=row_scale_init, col_vec=col_vec_init, heavy=heavy,
+                                    avg_degree=art.level_avg_degree.get((cfg, l), 1.0) if segs else 1.0))
+            for i, step in enumerate(streams[l]):
+                flush()
+                slot_ptr = self._slot_indptr[step.slot].data_ptr()
+                slot_idx = self._slot_indices[step.slot].data_ptr()
+                args = self._args(rows=(step.row_lo, step.row_hi), k=k, out=V, ld_out=ldv,
+                                  segs=[(slot_ptr, slot_idx, V + step.src_offset * ldv * w, ldv, step.row_lo)],
+                                  accumulate=True, avg_degree=step.nnz / max(1, step.row_hi - step.row_lo))
+                items.append(("stream", step, args, events[l][i], (l, i)))
+        # write-out
+        if spec.emit_all_nodes:
+            batch.append(self._args(rows=(0, st.num_nodes), k=k, out=self._aux.data_ptr(), ld_out=max_k,
+                                    maps=[(D["inv_perm"].data_ptr(), V, ldv, 0, st.num_nodes)]))
+        elif spec.direction == Direction.UP:
+            rs = cv = None
+            if spec.apply_endpoint_bias:
+                if spec.init_mode == InitMode.XTX:
+                    if "xtx_up" not in D:
+                        raise RuntimeError("missing HIP up XTX endpoint bias")
+                    rs = (D["xtx_up"].data_ptr(), 0)
+                else:
+                    if "vec_up" not in D:
+                        raise RuntimeError("missing HIP up vector endpoint bias")
+                    rs, cv = (D["vec_up"].data_ptr(), 0), io1
+            batch.append(self._args(rows=(0, st.num_mutations), k=k, out=io0, ld_out=max_k,
+                                    segs=[(D["sel_ptr"].data_ptr(), D["sel_idx"].data_ptr(), V, ldv, 0)],
+                                    row_scale=rs, col_vec=cv))
+            if spec.use_miss:
+                batch.append(self._args(rows=(0, st.num_mutations), k=k, out=io1, ld_out=max_k,
+                                        segs=[(D["miss_ptr"].data_ptr(), D["miss_idx"].data_ptr(), V, ldv, 0)]))
+        else:
+            rs = cv = None
+            suffix = "_ind" if spec.by_individual else ""
+            if spec.apply_endpoint_bias:
+                key = ("xtx_dn" if spec.init_mode == InitMode.XTX else "vec_dn") + suffix
+                if key not in D:
+                    raise RuntimeError(f"missing HIP down {spec.init_mode.value} endpoint bias")
+                rs = (D[key].data_ptr(), 0)
+                cv = io1 if spec.init_mode == InitMode.VECTOR else None
+            if spec.by_individual:
+                batch.append(self._args(rows=(0, st.num_individuals), k=k, out=io0, ld_out=max_k,
+                                        segs=[(D["ind_ptr"].data_ptr(), D["ind_idx"].data_ptr(), V, ldv, 0)],
+                                        row_scale=rs, col_vec=cv, avg_degree=float(st.ploidy)))
+            else:
+                batch.append(self._args(rows=(0, st.num_samples), k=k, out=io0, ld_out=max_k,
+                                        maps=[(0, V, ldv, 0, st.num_samples)], row_scale=rs, col_vec=cv))
+        flush()
+        return items
+
+    # ---- execution -------------------------------------------------------
+
+    def _execute(self, art: _HipArtifact, spec: _CudaMatmulSpec, items) -> None:
+        torch = _torch()
+        with self._caller_root_scope():
+            root = self._root_stream
+            root_ptr = int(root.cuda_stream)
+            has_stream = any(it[0] == "stream" for it in items)
+            if has_stream:
+                with torch.cuda.stream(root):
+                    self._launch_event.record(root)
+                for cs in self._slot_copy_streams:
+                    cs.wait_event(self._launch_event)
+            for it in items:
+                if it[0] == "batch":
+                    self._lib.gather_batch(it[1], it[2], root_ptr)
+                    continue
+                _, step, args, (copy_done, compute_done), _key = it
+                cs = self._slot_copy_streams[step.slot]
+                with torch.cuda.stream(cs):
+                    if step.prev_in_slot is not None:
+                        pl, pi = step.prev_in_slot
+                        ev_list = art.up_events if spec.direction == Direction.UP else art.down_events
+                        cs.wait_event(ev_list[pl][pi][1])
+                    n_idx = int(step.host_indices.numel())
+                    n_ptr = int(step.host_indptr.numel())
+                    self._slot_indices[step.slot][:n_idx].copy_(step.host_indices, non_blocking=True)
+                    self._slot_indptr[step.slot][:n_ptr].copy_(step.host_indptr, non_blocking=True)
+                    copy_done.record(cs)
+                root.wait_event(copy_done)
+                self._lib.gather(args, root_ptr)
+                with torch.cuda.stream(root):
+                    compute_done.record(root)
+
+
+__all__ = ["HipRuntime", "launch_shape"]

@@ -1,0 +1,122 @@
+/* MI355X (gfx950) GRG matmul kernels -- C ABI of libgrg_hip.so.
+ *
+ * Drop-in boundary for pygrgl_spmv.backends.hip. The reference's GPU backends
+ * reach device code through these Python-side entry points, which this ABI
+ * replaces:
+ *   - per-block SpMM launches: pygrgl_spmv/backends/triton/kernel.py:141-206
+ *     (launch_block -> csr_spmm_add_kernel / csc_spmm_add_kernel) and
+ *     pygrgl_spmv/backends/cusparse/backend.py:1607-1624 (cusparseSpMM);
+ *   - the seed pass: pygrgl_spmv/backends/triton/backend.py:1116-1150
+ *     (_seed_state: zero, init, input / selector scatter, miss);
+ *   - the write-out pass: pygrgl_spmv/backends/triton/backend.py:1152-1219
+ *     (_apply_endpoint_bias, _write_output: selector gather, miss output,
+ *     endpoint bias, by_individual reduce, emit_all_nodes permutation).
+ * Here every one of those is a "row gather" launch: for each output row,
+ *   out[row, c] = (accumulate ? out[row, c] : 0)
+ *               + sum over CSR segments s of  sum_{j in ptr_s[row]..ptr_s[row+1]} src_s[idx_s[j], c]
+ *               + sum over map segments m with row in [lo_m, hi_m) of src_m[map_m[row - lo_m], c]
+ *               + (row_scale ? row_scale[row] : 1) * (col_vec ? col_vec[c] : 1)   (if either is set)
+ * with no atomics: the down sweep reads child-major adjacency (parents of a
+ * node) and the selector transposes, so every output row is written once.
+ *
+ * Conventions: all pointers are device pointers (or NULL where optional);
+ * sizes are element counts; matrices are row-major with a leading dimension
+ * (elements between rows), k columns contiguous. The caller owns all memory;
+ * nothing here allocates, and no global mutable state is kept. Launches are
+ * asynchronous on the given hipStream_t (0 = legacy default stream) and may
+ * be captured into a hipGraph. Functions return 0 on success and a non-zero
+ * status otherwise, with a message available from grg_hip_last_error()
+ * (thread-local).
+ */
+#ifndef GRG_HIP_H
+#define GRG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRG_HIP_ABI_VERSION 1
+
+enum GrgHipDtype {
+  GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
+  GRG_HIP_F32C = 1      /* FP32 values, compensated (two-float) accumulation per row */
+};
+
+#define GRG_HIP_MAX_SEGS 3
+#define GRG_HIP_MAX_MAPS 2
+
+/* sum_{j in [ptr[row - ptr_base], ptr[row - ptr_base + 1])} src[idx[j] * ld + c] */
+typedef struct GrgHipSeg {
+  const int32_t* ptr;
+  const int32_t* idx;
+  const void* src;
+  int64_t ld;
+  int64_t ptr_base;
+} GrgHipSeg;
+
+/* for row in [row_lo, row_hi): src[(map ? map[row - row_lo] : row - row_lo) * ld + c] */
+typedef struct GrgHipMap {
+  const int32_t* map;
+  const void* src;
+  int64_t ld;
+  int64_t row_lo;
+  int64_t row_hi;
+} GrgHipMap;
+
+typedef struct GrgHipGatherArgs {
+  int32_t dtype;        /* GrgHipDtype */
+  int32_t accumulate;   /* 1: add into the existing out rows */
+  int64_t row_begin;    /* rows [row_begin, row_end) are produced */
+  int64_t row_end;
+  int32_t k;            /* columns */
+  int32_t cpl;          /* columns per lane: 1, 2 or 4; k, every ld and every base must be multiples */
+  int32_t kl;           /* lanes across one column chunk (power of two, kl * cpl columns per chunk) */
+  int32_t es;           /* lanes splitting one row's entries (power of two, kl * es <= 64) */
+  void* out;            /* out row r lives at out + (r - out_base) * ld_out */
+  int64_t ld_out;
+  int64_t out_base;
+  int32_t nseg;
+  int32_t nmap;
+  GrgHipSeg seg[GRG_HIP_MAX_SEGS];
+  GrgHipMap map[GRG_HIP_MAX_MAPS];
+  const void* row_scale;  /* indexed by row - scale_base */
+  int64_t scale_base;
+  const void* col_vec;    /* indexed by column */
+  /* Rows whose total segment degree exceeds heavy_threshold are skipped by the
+   * sub-wave path and produced instead by one workgroup each, taken from
+   * heavy_rows[0..num_heavy) (global row ids; the caller lists exactly those
+   * rows). heavy_threshold < 0 disables the split. */
+  int64_t heavy_threshold;
+  const int32_t* heavy_rows;
+  int64_t num_heavy;
+} GrgHipGatherArgs;
+
+/* One row-gather launch (see above). Replaces one level of the reference's
+ * per-block launch loop (triton/backend.py:1030-1046) with a single fused
+ * launch over all resident source levels, and the seed / write-out passes. */
+int grg_hip_gather(const GrgHipGatherArgs* args, void* stream);
+
+/* n row-gather launches in order on one stream (a whole prepared sweep:
+ * seed-fused levels then write-out), from one host call. */
+int grg_hip_gather_batch(const GrgHipGatherArgs* args, int32_t n, void* stream);
+
+/* Thread-local description of the last failure ("" if none). */
+const char* grg_hip_last_error(void);
+
+/* GRG_HIP_ABI_VERSION of the loaded library. */
+int32_t grg_hip_abi_version(void);
+
+/* hipGetDeviceCount, for device-ordinal validation without torch
+ * (replaces parse_cuda_device's cudaGetDeviceCount, backends/_cuda_stream.py:121-124). */
+int grg_hip_device_count(int32_t* count);
+
+/* hipStreamGetDevice (replaces backends/_cuda_stream.py:127-137). */
+int grg_hip_stream_device(void* stream, int32_t* device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRG_HIP_H */
