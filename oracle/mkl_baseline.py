@@ -39,12 +39,18 @@ def _load_mkl():
         if not cand:
             continue
         try:
-            lib = ctypes.CDLL(cand)
+            # DEEPBIND: libtorch exports MKL symbols of its own; MKL must bind to itself.
+            lib = ctypes.CDLL(cand, mode=os.RTLD_LOCAL | getattr(os, "RTLD_DEEPBIND", 0))
         except OSError:
             continue
         lib.MKL_Set_Interface_Layer.argtypes = [c_int]
         lib.MKL_Set_Interface_Layer.restype = c_int
         lib.MKL_Set_Interface_Layer(0)  # LP64: 32-bit MKL_INT
+        # GNU OpenMP threading: torch already loaded libgomp, and a second
+        # OpenMP runtime (Intel's) in the same process crashes.
+        lib.MKL_Set_Threading_Layer.argtypes = [c_int]
+        lib.MKL_Set_Threading_Layer.restype = c_int
+        lib.MKL_Set_Threading_Layer(3)  # MKL_THREADING_GNU
         lib.mkl_sparse_d_create_csr.argtypes = [POINTER(c_void_p), c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
         lib.mkl_sparse_d_mm.argtypes = [c_int, c_double, c_void_p, _Descr, c_int, c_void_p, c_int, c_int, c_double, c_void_p, c_int]
         lib.mkl_sparse_d_mv.argtypes = [c_int, c_double, c_void_p, _Descr, c_void_p, c_double, c_void_p]

@@ -68,8 +68,15 @@ def synthetic_state(*, blocks, level_offsets, num_samples, num_mutations, num_no
 
 
 def _band_csr(n, bw, shift, dtype):
+    """Binary CSR that keeps the given structural dtype (scipy would narrow int64 to int32)."""
+    import scipy.sparse as sp
+
     idx, ptr = band_block(n, bw, shift, dtype)
-    return binary_csr_from_parts(indices=idx, indptr=ptr, shape=(n, n), shared_data=True)
+    m = sp.csr_matrix((n, n), dtype=np.bool_)
+    m.data = np.ones(idx.size, dtype=np.bool_)
+    m.indices = idx
+    m.indptr = ptr
+    return m
 
 
 def write_band_artifact(artifact_dir, name, *, n, bandwidth, levels=3, struct_dtype=np.int32, shifts=None) -> Path:
