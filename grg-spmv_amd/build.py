@@ -59,8 +59,17 @@ def build_synth() -> Path:
     return target
 
 
+def build_host() -> Path:
+    LIB.mkdir(parents=True, exist_ok=True)
+    target = LIB / "libgrg_host.so"
+    cxx = shutil.which("g++") or "g++"
+    _atomic_build(lambda out: [cxx, "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+                               f"-I{INCLUDE}", "-o", str(out), str(CSRC / "grg_host.cpp")], target)
+    return target
+
+
 def build_all() -> list[Path]:
-    return [build_hip(), build_synth()]
+    return [build_hip(), build_synth(), build_host()]
 
 
 if __name__ == "__main__":

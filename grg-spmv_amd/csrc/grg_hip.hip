@@ -7,14 +7,27 @@
 // 0/1, so there are no multiplies), seeded by fused terms, and is written
 // exactly once: no atomics anywhere.
 //
-// Work mapping (light path): a wave of 64 lanes is split into groups of
-// G = kl * es lanes, one row per group. kl lanes cover a chunk of kl * cpl
-// columns with cpl-wide vector loads (16 bytes per lane for FP64 cpl=2 or FP32
-// cpl=4), and es lanes split the row's entries; the es partial sums are
-// combined with cross-lane xor shuffles. Rows whose degree exceeds a
-// threshold are skipped there and produced instead by the heavy path in the
-// same launch: one 256-thread workgroup per row, es = 256 / kl, with a
-// shuffle + LDS reduction. Grid y covers column chunks when k > kl * cpl.
+// The sweep is bound by the latency of dependent gathers (row pointer ->
+// column index -> node row), not by bytes, so the kernel is built to keep
+// many independent gathers in flight with few registers:
+//
+// Light path. A wave of 64 lanes is split into lane groups of G = kl * es
+// lanes, one row per group; kl lanes cover a chunk of kl * cpl columns with
+// cpl-wide vector loads, es lanes interleave over the row's entries (so the
+// group's lanes read adjacent entries, which the locality order of the node
+// ids keeps close in memory). Each lane works in rounds of kUnroll
+// predicated gathers: all index loads of the round, then all row loads, then
+// the adds, so a row of up to kUnroll * es entries costs one round trip.
+// The es partial sums are combined with cross-lane xor shuffles.
+//
+// Heavy path (same launch): rows of high degree get one 256-thread
+// workgroup each (256 / kl edge lanes), reduced with shuffles plus one LDS
+// pass.
+//
+// Optional XCD remap: workgroups are dispatched round-robin over the 8 XCDs,
+// so consecutive row blocks would land on different L2s; the remap gives each
+// XCD one contiguous eighth of the rows, keeping neighbouring rows (which
+// share children in the locality order) on one L2.
 //
 // FP32 values use compensated (TwoSum) accumulation per row; FP64 values use
 // plain FP64 adds.
@@ -35,8 +48,9 @@ int fail(const std::string& msg, int code = 1) {
   return code;
 }
 
-constexpr int kBlock = 256;            // threads per workgroup
+constexpr int kBlock = 256;  // threads per workgroup
 constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int kUnroll = 8;   // gathers in flight per lane and round
 
 template <typename T>
 struct PlainAcc {
@@ -107,48 +121,36 @@ __device__ __forceinline__ void store_cols(T* p, const T (&v)[CPL]) {
   }
 }
 
-// Sum src rows idx[j] for j = first, first + stride, ... < end into acc.
+// Sum src[idx[j]] for j = first, first + stride, ... < end into acc, in rounds
+// of kUnroll predicated gathers (absent entries add an exact zero).
 template <typename T, int CPL, typename Acc>
-__device__ __forceinline__ void gather_range(const int32_t* __restrict__ idx, const T* __restrict__ src, int64_t ld,
-                                             int32_t first, int32_t end, int32_t stride, Acc (&acc)[CPL]) {
-  int32_t j = first;
-  for (; j + 3 * stride < end; j += 4 * stride) {
-    const int64_t i0 = idx[j], i1 = idx[j + stride], i2 = idx[j + 2 * stride], i3 = idx[j + 3 * stride];
-    T v0[CPL], v1[CPL], v2[CPL], v3[CPL];
-    load_cols<T, CPL>(src + i0 * ld, v0);
-    load_cols<T, CPL>(src + i1 * ld, v1);
-    load_cols<T, CPL>(src + i2 * ld, v2);
-    load_cols<T, CPL>(src + i3 * ld, v3);
+__device__ __forceinline__ void gather_entries(const int32_t* __restrict__ idx, const T* __restrict__ src, int64_t ld,
+                                               int32_t first, int32_t end, int32_t stride, Acc (&acc)[CPL]) {
+  for (int32_t j = first; j < end; j += kUnroll * stride) {
+    int32_t ii[kUnroll];
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      acc[c].add(v0[c]);
-      acc[c].add(v1[c]);
-      acc[c].add(v2[c]);
-      acc[c].add(v3[c]);
+    for (int t = 0; t < kUnroll; ++t) {
+      const int32_t jt = j + t * stride;
+      ii[t] = jt < end ? idx[jt] : -1;
     }
-  }
-  for (; j < end; j += stride) {
-    const int64_t i0 = idx[j];
-    T v0[CPL];
-    load_cols<T, CPL>(src + i0 * ld, v0);
+    T v[kUnroll][CPL];
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) acc[c].add(v0[c]);
+    for (int t = 0; t < kUnroll; ++t) {
+      if (ii[t] >= 0) {
+        load_cols<T, CPL>(src + (int64_t)ii[t] * ld, v[t]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) v[t][c] = T(0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kUnroll; ++t)
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) acc[c].add(v[t][c]);
   }
 }
 
-// Every CSR segment's entries j = lane_first, lane_first + stride, ... of row.
-template <typename T, int CPL, typename Acc>
-__device__ __forceinline__ void gather_segments(const GrgHipGatherArgs& a, int64_t row, int col, int32_t lane_first,
-                                                int32_t stride, Acc (&acc)[CPL]) {
-  for (int s = 0; s < a.nseg; ++s) {
-    const GrgHipSeg& sg = a.seg[s];
-    const int32_t* ptr = sg.ptr + (row - sg.ptr_base);
-    const int32_t p0 = ptr[0], p1 = ptr[1];
-    gather_range<T, CPL>(sg.idx, static_cast<const T*>(sg.src) + col, sg.ld, p0 + lane_first, p1, stride, acc);
-  }
-}
-
-// Row-wise seed / epilogue terms (added once per row).
+// Row-wise seed / epilogue terms (added once per row by one lane).
 template <typename T, int CPL, typename Acc>
 __device__ __forceinline__ void add_row_terms(const GrgHipGatherArgs& a, int64_t row, int col, Acc (&acc)[CPL]) {
   for (int m = 0; m < a.nmap; ++m) {
@@ -203,20 +205,31 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
     const int lg = lane & (G - 1);
     const int e = lg >> kl_shift;
     const int ES = G >> kl_shift;
-    const int64_t row = a.row_begin + ((((int64_t)blockIdx.x * kWavesPerBlock + wave) << (6 - g_shift)) + (lane >> g_shift));
     const int col = ((int)blockIdx.y * KL + (lg & (KL - 1))) * CPL;
+    int64_t blk = blockIdx.x;
+    if (a.flags & GRG_HIP_FLAG_XCD_REMAP) {
+      // block b runs on XCD b % 8; give XCD x the x-th contiguous eighth of the rows
+      const int64_t q = n_light_blocks >> 3, r = n_light_blocks & 7, x = blk & 7;
+      blk = x * q + (x < r ? x : r) + (blk >> 3);
+    }
+    const int64_t row = a.row_begin + (((blk * kWavesPerBlock + wave) << (6 - g_shift)) + (lane >> g_shift));
     bool live = row < a.row_end && col < a.k;
-    if (live && a.heavy_threshold >= 0) {
+    if (live) {
+      int32_t p0[GRG_HIP_MAX_SEGS], p1[GRG_HIP_MAX_SEGS];
       int64_t deg = 0;
       for (int s = 0; s < a.nseg; ++s) {
         const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
-        deg += ptr[1] - ptr[0];
+        p0[s] = ptr[0];
+        p1[s] = ptr[1];
+        deg += p1[s] - p0[s];
       }
-      live = deg <= a.heavy_threshold;
-    }
-    if (live) {
-      gather_segments<T, CPL>(a, row, col, e, ES, acc);
-      if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
+      if (a.heavy_threshold >= 0 && deg > a.heavy_threshold) {
+        live = false;  // produced by the heavy path
+      } else {
+        for (int s = 0; s < a.nseg; ++s)
+          gather_entries<T, CPL>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, p0[s] + e, p1[s], ES, acc);
+        if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
+      }
     }
     for (int off = KL; off < G; off <<= 1) {
 #pragma unroll
@@ -234,7 +247,10 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
   const int col = ((int)blockIdx.y * KL + cl) * CPL;
   const bool live = col < a.k;
   if (live) {
-    gather_segments<T, CPL>(a, row, col, e, ES, acc);
+    for (int s = 0; s < a.nseg; ++s) {
+      const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
+      gather_entries<T, CPL>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, ptr[0] + e, ptr[1], ES, acc);
+    }
     if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
   }
   for (int off = KL; off < 64; off <<= 1) {
@@ -277,6 +293,7 @@ int validate(const GrgHipGatherArgs& a) {
   if (a.cpl != 1 && a.cpl != 2 && a.cpl != 4) return fail("cpl must be 1, 2 or 4");
   if (!is_pow2(a.kl) || !is_pow2(a.es) || a.kl * a.es > 64) return fail("kl and es must be powers of two with kl * es <= 64");
   if (a.kl * a.cpl > 64) return fail("kl * cpl must be <= 64");
+  if (a.flags & ~GRG_HIP_FLAG_XCD_REMAP) return fail("unknown flags");
   if (a.k % a.cpl) return fail("k must be a multiple of cpl");
   if (!a.out) return fail("out is null");
   if (a.nseg < 0 || a.nseg > GRG_HIP_MAX_SEGS) return fail("nseg out of range");
@@ -291,8 +308,8 @@ int validate(const GrgHipGatherArgs& a) {
     if (!a.map[m].src) return fail("map " + std::to_string(m) + " has a null src");
     if (misaligned(a.map[m].src, a.map[m].ld, a.cpl, elem)) return fail("map src / ld not aligned to cpl columns");
   }
-  if (a.num_heavy < 0 || (a.num_heavy > 0 && (!a.heavy_rows || a.heavy_threshold < 0)))
-    return fail("heavy rows need heavy_rows and heavy_threshold >= 0");
+  if (a.num_heavy < 0 || (a.num_heavy > 0 && !a.heavy_rows)) return fail("heavy rows missing");
+  if (a.num_heavy > 0 && a.heavy_threshold < 0) return fail("heavy rows need heavy_threshold >= 0");
   return 0;
 }
 
@@ -303,11 +320,10 @@ void launch_typed(const GrgHipGatherArgs& a, dim3 grid, int64_t n_light, int g_s
 
 int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
   if (int rc = validate(a)) return rc;
-  const int64_t rows = a.row_end - a.row_begin;
   const int G = a.kl * a.es;
   const int g_shift = ilog2(G), kl_shift = ilog2(a.kl);
   const int64_t rows_per_block = (int64_t)kWavesPerBlock * (64 / G);
-  const int64_t n_light = (rows + rows_per_block - 1) / rows_per_block;
+  const int64_t n_light = (a.row_end - a.row_begin + rows_per_block - 1) / rows_per_block;
   const int64_t gx = n_light + a.num_heavy;
   if (gx == 0) return 0;
   if (gx > 0x7fffffffLL) return fail("grid too large");

@@ -13,9 +13,10 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 1
+ABI_VERSION = 3
 F64 = 0
 F32C = 1
+FLAG_XCD_REMAP = 2
 MAX_SEGS = 3
 MAX_MAPS = 2
 
@@ -38,6 +39,7 @@ class GrgHipGatherArgs(ctypes.Structure):
         ("cpl", c_int32),
         ("kl", c_int32),
         ("es", c_int32),
+        ("flags", c_int32),
         ("out", c_void_p),
         ("ld_out", c_int64),
         ("out_base", c_int64),

@@ -16,8 +16,8 @@ What differs is what is budgeted, because the HIP runtime allocates different
 things: both sweep orientations are stored (no shared storage), selectors are
 int32 CSR (forward for the up write-out, transposed for the down seed), the
 resident blocks of each destination level are merged into one CSR whose row
-pointers are counted under ``metadata``, and heavy-row lists get their own
-bounded category. ``bytes_total`` is an upper bound of what ``HipRuntime``
+pointers are counted under ``metadata``, and the per-level row-batch tables
+and heavy-row lists get their own bounded category (``schedule``). ``bytes_total`` is an upper bound of what ``HipRuntime``
 allocates for the layout.
 """
 
@@ -36,8 +36,9 @@ from pygrgl_spmv.backends.types import Direction
 from pygrgl_spmv.grg import RuntimeRequirements
 from pygrgl_spmv.grg.artifact import ArtifactScan, scan_grg_spmv
 
-# A row is "heavy" (one workgroup of its own) when its gather degree exceeds this.
-HEAVY_THRESHOLD = 256
+# Rows of gather degree above this (or above 8x their level's typical batch) get a
+# workgroup of their own.
+HEAVY_THRESHOLD = 512
 
 
 @dataclass
@@ -110,7 +111,7 @@ class HipLayout:
     bytes_by_category: dict = field(default_factory=dict)
     bytes_total: int = 0
     heavy_threshold: int = HEAVY_THRESHOLD
-    heavy_capacity: tuple[int, ...] = ()
+    schedule_capacity: tuple[int, ...] = ()
 
 
 def _resolve_artifacts(artifacts) -> tuple[Path, ...]:
@@ -212,7 +213,7 @@ def _selector_bytes(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequireme
 def _metadata_bytes(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequirements, dtype: np.dtype) -> int:
     S, M, N, I = scan.num_samples, scan.num_mutations, scan.num_nodes, scan.num_individuals
     w = dtype.itemsize
-    total = 4 * (2 * N + S + I + 1 + S)  # node_perm, inv_node_perm, sample_to_individual, individual -> samples CSR
+    total = 4 * (2 * N + 3 * S + I + 1 + S)  # node maps (2N), sample input/output maps (3S), individual -> samples CSR
     total += 4 * (N + 1) * ((pair.plan_up is not None) + (pair.plan_down is not None))  # merged level row pointers
     if req.need_init_vector:
         total += w * ((M if pair.plan_up is not None else 0) + ((S + I) if pair.plan_down is not None else 0))
@@ -221,17 +222,15 @@ def _metadata_bytes(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequireme
     return int(total)
 
 
-def heavy_capacity_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequirements, threshold: int) -> int:
-    """Entries reserved for heavy-row lists: rows of degree > threshold are at most total degree / (threshold + 1)."""
-    bound = lambda total: min(scan.num_nodes, total // (threshold + 1) + 1)  # noqa: E731
-    n = 0
-    if pair.plan_up is not None:
-        n += bound(scan.num_edges)
-    if pair.plan_down is not None:
-        n += bound(scan.num_edges + scan.selector_mut_nnz)
-        if req.need_down_miss_input:
-            n += bound(scan.num_edges + scan.selector_mut_nnz + scan.selector_miss_nnz)
-    return int(n)
+def schedule_capacity_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequirements) -> int:
+    """int32 entries reserved for row-batch tables and heavy-row lists.
+
+    Every sweep configuration (up; down; down with missing-data input) lists
+    each node either inside one batch or as a heavy row, and a batch holds at
+    least one row, so one configuration needs at most ``num_nodes`` entries.
+    """
+    configs = (pair.plan_up is not None) + (pair.plan_down is not None) * (2 if req.need_down_miss_input else 1)
+    return int(configs * scan.num_nodes)
 
 
 def plan_hip_layout(
@@ -282,7 +281,7 @@ def plan_hip_layout(
         metadata_bytes += _metadata_bytes(s, pair, requirements, dtype)
         if requirements.need_init_xtx and s.has_individual_coals:
             xtx_bytes += s.num_nodes * dtype.itemsize
-        cap = heavy_capacity_for(s, pair, requirements, HEAVY_THRESHOLD)
+        cap = schedule_capacity_for(s, pair, requirements)
         heavy_caps.append(cap)
         heavy_bytes += 4 * cap
     max_k = enabled_max_k(pair, requirements)
@@ -357,13 +356,13 @@ def plan_hip_layout(
         "selectors": int(selector_bytes),
         "metadata": int(metadata_bytes),
         "xtx_bias": int(xtx_bytes),
-        "heavy_rows": int(heavy_bytes),
+        "schedule": int(heavy_bytes),
         "dense_arena": int(dense_bytes),
         "state": int(state_bytes),
         "scratch": 0,
     }
     items = [BudgetItem(kind="fixed", name=name, nbytes=int(by_cat[name]))
-             for name in ("selectors", "metadata", "xtx_bias", "heavy_rows", "dense_arena", "state", "scratch")]
+             for name in ("selectors", "metadata", "xtx_bias", "schedule", "dense_arena", "state", "scratch")]
     for ai, al in enumerate(planned):
         for b in (*al.blocks_up, *al.blocks_down):
             if b.resident:
@@ -380,7 +379,7 @@ def plan_hip_layout(
         max_up_ops_by_level=max_up_ops, max_down_ops_by_level=max_down_ops, slot_plans=slots,
         budget_items=tuple(i for i in items if i.nbytes > 0), required_budget_for_full_residency=int(required_full),
         bytes_by_category=by_cat, bytes_total=int(sum(i.nbytes for i in items)),
-        heavy_threshold=HEAVY_THRESHOLD, heavy_capacity=tuple(heavy_caps),
+        heavy_threshold=HEAVY_THRESHOLD, schedule_capacity=tuple(heavy_caps),
     )
 
 

@@ -32,8 +32,8 @@ Execution differs from the reference by design:
 from __future__ import annotations
 
 from contextlib import contextmanager
-from dataclasses import dataclass, field, replace
-import math
+import ctypes
+from dataclasses import dataclass, field
 from pathlib import Path
 
 import numpy as np
@@ -66,19 +66,56 @@ def _pow2_floor(v: float) -> int:
     return 1 if v < 2 else 1 << (int(v).bit_length() - 1)
 
 
-def launch_shape(dtype_code: int, k: int, lds, avg_degree: float, *, es_cap: int = 16) -> tuple[int, int, int]:
-    """(cpl, kl, es): 16-byte vector columns when k and every leading dimension allow,
-    kl lanes to cover the k columns (at most 64 columns per chunk), and es lanes per row
-    so that each lane handles about two entries on average."""
-    cands = (4, 2, 1) if dtype_code == ffi.F32C else (2, 1)
-    cpl = 1
-    for c in cands:
-        if k % c == 0 and all(int(ld) % c == 0 for ld in lds):
-            cpl = c
-            break
-    kl = min(_pow2_ceil(-(-k // cpl)), 64 // cpl)
-    es = max(1, min(_pow2_floor(max(avg_degree, 1.0) / 2.0), 64 // kl, es_cap))
+KERNEL_UNROLL = 8  # gathers in flight per lane and round in the kernel
+
+
+def column_shapes(dtype_code: int, k: int, lds) -> list[tuple[int, int]]:
+    """Admissible (cpl, kl): cpl-wide vector columns (up to 32 bytes per lane) that
+    divide k and every leading dimension, kl lanes covering k (<= 64 columns per chunk)."""
+    elem = 8 if dtype_code == ffi.F64 else 4
+    out = []
+    for cpl in (4, 2, 1):
+        if cpl * elem > 32 or k % cpl or any(int(ld) % cpl for ld in lds):
+            continue
+        out.append((cpl, min(_pow2_ceil(-(-k // cpl)), 64 // cpl)))
+    return out or [(1, min(_pow2_ceil(k), 64))]
+
+
+def launch_shape(dtype_code: int, k: int, lds, mean_degree: float) -> tuple[int, int, int]:
+    """Default (cpl, kl, es) before tuning: 16-byte columns, and edge lanes so one
+    round of KERNEL_UNROLL gathers per lane covers a typical row."""
+    shapes = column_shapes(dtype_code, k, lds)
+    elem = 8 if dtype_code == ffi.F64 else 4
+    cpl, kl = next(((c, l) for c, l in shapes if c * elem <= 16), shapes[-1])
+    es = max(1, min(_pow2_floor(max(mean_degree, 1.0) / KERNEL_UNROLL), 64 // kl))
     return cpl, kl, es
+
+
+@dataclass(frozen=True)
+class _LevelSchedule:
+    """Heavy rows of one level launch: rows of gather degree > ``heavy_threshold``
+    are produced by a workgroup each and listed at ``heavy_offset`` in the
+    schedule buffer; ``mean_degree`` is the mean over the other rows."""
+
+    heavy_offset: int
+    num_heavy: int
+    heavy_threshold: int
+    mean_degree: float
+
+
+def level_schedule(deg: np.ndarray, lo: int, hi: int, *, heavy_min: int, cursor: int):
+    """Schedule for rows [lo, hi) of one sweep configuration (``deg`` = gather degree per row).
+
+    Rows above max(heavy_min, 8 x the level's mean degree) get a 256-thread
+    workgroup of their own, so one long row does not hold a wave of short ones.
+    """
+    seg = deg[lo:hi]
+    mean = float(seg.mean()) if seg.size else 0.0
+    heavy = int(max(heavy_min, 8.0 * mean))
+    heavy_rows = (np.flatnonzero(seg > heavy) + lo).astype(np.int32)
+    light = seg[seg <= heavy]
+    light_mean = max(1.0, float(light.mean())) if light.size else 1.0
+    return _LevelSchedule(cursor, int(heavy_rows.size), heavy, light_mean), heavy_rows
 
 
 # --------------------------------------------------------------------------
@@ -106,8 +143,7 @@ class _HipArtifact:
     state: object
     level_offsets: np.ndarray
     dev: dict = field(default_factory=dict)           # name -> device tensor
-    level_avg_degree: dict = field(default_factory=dict)  # (config, level) -> float
-    heavy: dict = field(default_factory=dict)         # (config, level) -> (offset, count)
+    sched: dict = field(default_factory=dict)         # (config, level) -> _LevelSchedule
     up_stream: list = field(default_factory=list)     # per level: list[_StreamStep]
     down_stream: list = field(default_factory=list)
     up_events: list = field(default_factory=list)     # per level: list[(copy_done, compute_done)]
@@ -126,6 +162,24 @@ def _csr_transpose(indptr, indices, n_rows: int, n_cols: int):
     t = m.T.tocsr()
     t.sort_indices()
     return np.asarray(t.indptr, dtype=np.int64), np.asarray(t.indices, dtype=np.int64)
+
+
+def _remapped_block(indptr, indices, new_pos: np.ndarray, off: np.ndarray, d: int, s: int, *, transpose: bool):
+    """A block's CSR in the device node order: rows of level d and columns of level s
+    (or, with ``transpose``, rows of level s and columns of level d), level-local, sorted."""
+    rows_d = int(off[d + 1] - off[d])
+    rows_s = int(off[s + 1] - off[s])
+    ptr = np.asarray(indptr, dtype=np.int64)
+    r = np.repeat(np.arange(rows_d, dtype=np.int64), np.diff(ptr))
+    c = np.asarray(indices, dtype=np.int64)
+    r_new = new_pos[r + off[d]] - off[d]
+    c_new = new_pos[c + off[s]] - off[s]
+    if transpose:
+        m = sp.csr_matrix((np.ones(r.size, dtype=np.int8), (c_new, r_new)), shape=(rows_s, rows_d))
+    else:
+        m = sp.csr_matrix((np.ones(r.size, dtype=np.int8), (r_new, c_new)), shape=(rows_d, rows_s))
+    m.sort_indices()
+    return np.asarray(m.indptr, dtype=np.int64), np.asarray(m.indices, dtype=np.int64)
 
 
 class _HipPreparedMatmul:
@@ -177,7 +231,7 @@ class _HipPreparedMatmul:
 class HipRuntime:
     """Runtime-owned HIP execution with resident or streamed sparse blocks."""
 
-    def __init__(self, layout: HipLayout) -> None:
+    def __init__(self, layout: HipLayout, *, autotune: bool = True) -> None:
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError("HIP runtime requires a ROCm GPU (torch.cuda.is_available() is False)")
@@ -185,6 +239,7 @@ class HipRuntime:
         self.layout = layout
         self.device = torch.device("cuda", int(layout.device))
         self.stream_ptr = int(layout.stream_ptr)
+        self.autotune = bool(autotune)
         self._dtype_code = ffi.F64 if np.dtype(layout.dtype) == np.float64 else ffi.F32C
         self._reset_owned_state()
 
@@ -206,6 +261,7 @@ class HipRuntime:
         self._aux = None
         self._artifacts = ()
         self._grgs = ()
+        self._tune_cache = {}
         self._entered = False
         self._active_call = False
 
@@ -316,47 +372,42 @@ class HipRuntime:
             copy_struct_checked(host.numpy()[: np.asarray(values).size], np.asarray(values), label="streamed block structure")
         return host
 
-    def _merge_levels(self, parts_by_level, offsets: np.ndarray):
-        """Concatenate per-level CSR parts (global column ids) into one CSR over all rows.
+    def _merge_parts(self, N: int, parts):
+        """One CSR over all N rows from parts (row_ids int64[rows], indptr int64[rows + 1], cols int32[nnz]).
 
-        ``parts_by_level[l]`` is a list of (indptr int64[rows_l + 1], cols int32[nnz]) device
-        tensors whose rows are the nodes of level l; parts are laid out in list order within
-        each row. Returns (ptr int32[N + 1], idx int32[max(E, 1)], degree int64[N]) on device.
+        ``row_ids`` are the (new) global rows of a part's local rows; entries keep
+        their order within a row and parts are laid out in list order within each
+        row. Returns (ptr int32[N + 1], idx int32[max(E, 1)], degree int64[N]) on device.
         """
         torch = _torch()
         dev = self.device
-        N = int(offsets[-1])
         deg = torch.zeros(N, dtype=torch.int64, device=dev)
-        for l, parts in enumerate(parts_by_level):
-            lo, hi = int(offsets[l]), int(offsets[l + 1])
-            for ptr, _ in parts:
-                deg[lo:hi] += ptr[1:] - ptr[:-1]
+        for rows, ptr, _ in parts:
+            deg.index_add_(0, rows, ptr[1:] - ptr[:-1])
         ptr_all = torch.zeros(N + 1, dtype=torch.int64, device=dev)
         torch.cumsum(deg, 0, out=ptr_all[1:])
         total = int(ptr_all[-1].item())
         if total > np.iinfo(np.int32).max:
             raise NotImplementedError("more than 2^31 - 1 resident entries in one sweep orientation")
         idx = torch.zeros(max(total, 1), dtype=torch.int32, device=dev)
-        for l, parts in enumerate(parts_by_level):
-            lo, hi = int(offsets[l]), int(offsets[l + 1])
-            cursor = ptr_all[lo:hi].clone()
-            rows = hi - lo
-            for ptr, cols in parts:
-                d = ptr[1:] - ptr[:-1]
-                nnz = int(cols.numel())
-                if nnz == 0:
-                    continue
-                row_of = torch.repeat_interleave(torch.arange(rows, device=dev), d)
-                pos = cursor[row_of] + (torch.arange(nnz, device=dev) - ptr[:-1][row_of])
-                idx[pos] = cols
-                cursor += d
+        cursor = ptr_all[:-1].clone()
+        for rows, ptr, cols in parts:
+            d = ptr[1:] - ptr[:-1]
+            nnz = int(cols.numel())
+            if nnz == 0:
+                continue
+            local = torch.repeat_interleave(torch.arange(rows.numel(), device=dev), d)
+            pos = cursor[rows[local]] + (torch.arange(nnz, device=dev) - ptr[:-1][local])
+            idx[pos] = cols
+            cursor.index_add_(0, rows, d)
         return ptr_all.to(torch.int32), idx, deg
 
     def _build_artifact(self, ai: int, al, st) -> _HipArtifact:
+        from pygrgl_spmv.backends.hip.host import locality_order
+
         torch = _torch()
         lay = self.layout
         dev = self.device
-        w_dtype = self._torch_dtype()
         off = np.asarray(st.level_offsets, dtype=np.int64)
         L = off.size - 1
         S, M, N, I = st.num_samples, st.num_mutations, st.num_nodes, st.num_individuals
@@ -364,62 +415,69 @@ class HipRuntime:
         D = art.dev
         up_plan = {(b.dst_level, b.src_level): b for b in al.blocks_up}
         dn_plan = {(b.dst_level, b.src_level): b for b in al.blocks_down}
-        up_parts = [[] for _ in range(L)]   # by parent level, ascending source level
-        dn_parts = {l: [] for l in range(L)}  # by child level: (parent level, part)
+        host_blocks = []
+        for blk in iter_artifact_blocks(al.path):
+            if blk.nnz == 0:
+                continue
+            idx = np.asarray(blk.indices)
+            if idx.dtype != np.int32:
+                idx = _i32(idx, f"A_blocks[{blk.dst_level}][{blk.src_level}].indices")
+            host_blocks.append((blk.dst_level, blk.src_level, np.asarray(blk.indptr, dtype=np.int64), idx))
+        # Device-private node order: a permutation inside every level that keeps nodes
+        # sharing descendants adjacent (samples stay the prefix of level 0).
+        new_pos = locality_order(N, S, off, host_blocks)
+        new_pos_t = self._to_dev(new_pos)
+        up_parts, dn_parts = [], []
         up_stream = [[] for _ in range(L)]
         dn_stream = [[] for _ in range(L)]
         with torch.cuda.device(dev):
-            for blk in iter_artifact_blocks(al.path):
-                if blk.nnz == 0:
-                    continue
-                d, s = blk.dst_level, blk.src_level
-                ptr_h = np.asarray(blk.indptr, dtype=np.int64)
-                idx_h = np.asarray(blk.indices, dtype=np.int64)
+            for d, s, ptr_h, idx_h in host_blocks:
+                rows_d = int(off[d + 1] - off[d])
+                rows_s = int(off[s + 1] - off[s])
                 bp = up_plan.get((d, s))
                 if bp is not None:
                     if bp.resident:
                         ptr_d = self._to_dev(ptr_h)
-                        cols_d = (self._to_dev(idx_h) + int(off[s])).to(torch.int32)
-                        up_parts[d].append((ptr_d, cols_d))
+                        cols = new_pos_t[self._to_dev(idx_h).to(torch.int64) + int(off[s])].to(torch.int32)
+                        up_parts.append((d, s, (new_pos_t[int(off[d]) : int(off[d + 1])], ptr_d, cols)))
                     else:
+                        rp, ri = _remapped_block(ptr_h, idx_h, new_pos, off, d, s, transpose=False)
                         up_stream[d].append(_StreamStep(
-                            dst_level=d, src_level=s, slot=int(bp.slot), host_indices=self._pinned(idx_h),
-                            host_indptr=self._pinned(ptr_h), row_lo=int(off[d]), row_hi=int(off[d + 1]),
-                            src_offset=int(off[s]), nnz=int(idx_h.size)))
+                            dst_level=d, src_level=s, slot=int(bp.slot), host_indices=self._pinned(ri),
+                            host_indptr=self._pinned(rp), row_lo=int(off[d]), row_hi=int(off[d + 1]),
+                            src_offset=int(off[s]), nnz=int(ri.size)))
                 bp = dn_plan.get((d, s))
                 if bp is not None:
                     if bp.resident:
                         ptr_d = self._to_dev(ptr_h)
-                        cols_d = self._to_dev(idx_h)
-                        rows_d = int(off[d + 1] - off[d])
+                        cols_d = self._to_dev(idx_h).to(torch.int64)
                         row_of = torch.repeat_interleave(torch.arange(rows_d, device=dev), ptr_d[1:] - ptr_d[:-1])
                         keys, order = torch.sort(cols_d, stable=True)
-                        parents = (row_of[order] + int(off[d])).to(torch.int32)
-                        counts = torch.bincount(keys, minlength=int(off[s + 1] - off[s]))
-                        tptr = torch.zeros(counts.numel() + 1, dtype=torch.int64, device=dev)
+                        parents = new_pos_t[row_of[order] + int(off[d])].to(torch.int32)
+                        counts = torch.bincount(keys, minlength=rows_s)
+                        tptr = torch.zeros(rows_s + 1, dtype=torch.int64, device=dev)
                         torch.cumsum(counts, 0, out=tptr[1:])
-                        dn_parts[s].append((d, (tptr, parents)))
+                        dn_parts.append((d, s, (new_pos_t[int(off[s]) : int(off[s + 1])], tptr, parents)))
                     else:
-                        tptr_h, tidx_h = _csr_transpose(ptr_h, idx_h, int(off[d + 1] - off[d]), int(off[s + 1] - off[s]))
+                        rp, ri = _remapped_block(ptr_h, idx_h, new_pos, off, d, s, transpose=True)
                         dn_stream[s].append(_StreamStep(
-                            dst_level=s, src_level=d, slot=int(bp.slot), host_indices=self._pinned(tidx_h),
-                            host_indptr=self._pinned(tptr_h), row_lo=int(off[s]), row_hi=int(off[s + 1]),
-                            src_offset=int(off[d]), nnz=int(tidx_h.size)))
+                            dst_level=s, src_level=d, slot=int(bp.slot), host_indices=self._pinned(ri),
+                            host_indptr=self._pinned(rp), row_lo=int(off[s]), row_hi=int(off[s + 1]),
+                            src_offset=int(off[d]), nnz=int(ri.size)))
+            del host_blocks
+            deg_up = deg_dn = None
             if lay.pair.plan_up is not None:
-                D["up_ptr"], D["up_idx"], deg_up = self._merge_levels(up_parts, off)
-            else:
-                deg_up = None
+                parts = [p for _, _, p in sorted(up_parts, key=lambda t: (t[0], t[1]))]
+                D["up_ptr"], D["up_idx"], deg_up = self._merge_parts(N, parts)
             if lay.pair.plan_down is not None:
-                ordered = [[part for _, part in sorted(dn_parts[l], key=lambda t: -t[0])] for l in range(L)]
-                D["dn_ptr"], D["dn_idx"], deg_dn = self._merge_levels(ordered, off)
-            else:
-                deg_dn = None
+                parts = [p for _, _, p in sorted(dn_parts, key=lambda t: (t[1], -t[0]))]
+                D["dn_ptr"], D["dn_idx"], deg_dn = self._merge_parts(N, parts)
             del up_parts, dn_parts
-            # selectors
+            # selectors, in the new node ids
             sel_ptr = np.asarray(st.sel_mut.indptr, dtype=np.int64)
-            sel_idx = np.asarray(st.sel_mut.indices, dtype=np.int64)
+            sel_idx = new_pos[np.asarray(st.sel_mut.indices, dtype=np.int64)]
             miss_ptr = np.asarray(st.sel_miss.indptr, dtype=np.int64)
-            miss_idx = np.asarray(st.sel_miss.indices, dtype=np.int64)
+            miss_idx = new_pos[np.asarray(st.sel_miss.indices, dtype=np.int64)]
             req = lay.requirements
             if lay.pair.plan_up is not None:
                 D["sel_ptr"], D["sel_idx"] = self._to_dev(_i32(sel_ptr, "sel_mut.indptr")), self._to_dev(_i32(sel_idx, "sel_mut.indices"))
@@ -434,19 +492,27 @@ class HipRuntime:
                     tp, ti = _csr_transpose(miss_ptr, miss_idx, M, N)
                     D["missT_ptr"], D["missT_idx"] = self._to_dev(_i32(tp, "sel_miss^T.indptr")), self._to_dev(_i32(ti, "sel_miss^T.indices"))
                     deg_miss_t = np.diff(tp)
-            # per-node / per-sample metadata
-            node_perm = np.asarray(st.node_perm, dtype=np.int64)
+            # node / sample maps through the new order
+            node_perm = np.empty(N, dtype=np.int64)
+            node_perm[new_pos] = np.asarray(st.node_perm, dtype=np.int64)          # new internal -> original id
+            inv_perm = new_pos[np.asarray(st.inv_node_perm, dtype=np.int64)]        # original id -> new internal
             D["node_perm"] = self._to_dev(_i32(node_perm, "node_perm"))
-            D["inv_perm"] = self._to_dev(_i32(np.asarray(st.inv_node_perm, dtype=np.int64), "inv_node_perm"))
+            D["inv_perm"] = self._to_dev(_i32(inv_perm, "inv_node_perm"))
             s2i = np.asarray(st.sample_to_individual, dtype=np.int64)
-            D["s2i"] = self._to_dev(_i32(s2i, "sample_to_individual"))
+            sample_at = np.empty(S, dtype=np.int64)
+            sample_at[new_pos[:S]] = np.arange(S)                                    # new row -> sample id
+            D["in_map"] = self._to_dev(_i32(sample_at, "input map"))
+            D["in_map_ind"] = self._to_dev(_i32(s2i[sample_at], "input map (individuals)"))
+            D["out_map"] = self._to_dev(_i32(new_pos[:S], "output map"))
             order = np.argsort(s2i, kind="stable")
             ind_ptr = np.zeros(I + 1, dtype=np.int64)
             np.cumsum(np.bincount(s2i, minlength=I), out=ind_ptr[1:])
-            D["ind_ptr"], D["ind_idx"] = self._to_dev(_i32(ind_ptr, "individual.indptr")), self._to_dev(_i32(order, "individual.indices"))
+            D["ind_ptr"], D["ind_idx"] = self._to_dev(_i32(ind_ptr, "individual.indptr")), self._to_dev(_i32(new_pos[order], "individual.indices"))
             npdt = np.dtype(lay.dtype)
             if req.need_init_xtx and st.coalescence_counts is not None:
-                D["coal2"] = self._to_dev((2.0 * np.asarray(st.coalescence_counts, dtype=np.float64)).astype(npdt))
+                coal2 = np.empty(N, dtype=np.float64)
+                coal2[new_pos] = 2.0 * np.asarray(st.coalescence_counts, dtype=np.float64)
+                D["coal2"] = self._to_dev(coal2.astype(npdt))
             if req.need_init_vector:
                 if lay.pair.plan_up is not None:
                     D["vec_up"] = self._to_dev(np.asarray(st.init_vector_up_bias, dtype=npdt))
@@ -461,8 +527,7 @@ class HipRuntime:
                     xd = np.asarray(st.init_xtx_down_bias, dtype=np.float64)
                     D["xtx_dn"] = self._to_dev(xd.astype(npdt))
                     D["xtx_dn_ind"] = self._to_dev(np.bincount(s2i, weights=xd, minlength=I).astype(npdt))
-            # heavy rows and per-level average degrees for every sweep configuration
-            H = int(lay.heavy_threshold)
+            # row-batch tables and heavy rows for every sweep configuration
             configs = {}
             if deg_up is not None:
                 configs["up"] = deg_up.cpu().numpy()
@@ -471,21 +536,18 @@ class HipRuntime:
                 configs["down"] = base
                 if deg_miss_t is not None:
                     configs["down_miss"] = base + deg_miss_t
-            heavy_lists = []
+            tables = []
             cursor = 0
             for name, deg in configs.items():
                 for l in range(L):
-                    seg = deg[off[l] : off[l + 1]]
-                    rows = np.flatnonzero(seg > H) + off[l]
-                    art.heavy[(name, l)] = (cursor, int(rows.size))
-                    light = seg[seg <= H]
-                    art.level_avg_degree[(name, l)] = float(light.mean()) if light.size else 0.0
-                    heavy_lists.append(rows)
-                    cursor += int(rows.size)
-            cap = lay.heavy_capacity[ai] if ai < len(lay.heavy_capacity) else cursor
+                    sched, h = level_schedule(deg, int(off[l]), int(off[l + 1]), heavy_min=int(lay.heavy_threshold), cursor=cursor)
+                    art.sched[(name, l)] = sched
+                    tables.append(h)
+                    cursor += h.size
+            cap = lay.schedule_capacity[ai] if ai < len(lay.schedule_capacity) else cursor
             if cursor > cap:
-                raise RuntimeError(f"internal error: {cursor} heavy rows exceed the planned capacity {cap}")
-            D["heavy"] = self._to_dev(_i32(np.concatenate(heavy_lists) if heavy_lists else np.zeros(0, np.int64), "heavy rows"))
+                raise RuntimeError(f"internal error: {cursor} schedule entries exceed the planned capacity {cap}")
+            D["sched"] = self._to_dev(np.concatenate(tables).astype(np.int32) if tables else np.zeros(1, np.int32))
             # streamed ops with slot-reuse links, and their events
             art.up_stream = relink_stream_dependencies(up_stream, direction=Direction.UP)
             art.down_stream = relink_stream_dependencies(dn_stream, direction=Direction.DOWN)
@@ -499,14 +561,23 @@ class HipRuntime:
         return 8 if self._dtype_code == ffi.F64 else 4
 
     def _args(self, *, rows, k, out, ld_out, segs=(), maps=(), row_scale=None, col_vec=None, accumulate=False,
-              heavy=None, avg_degree=1.0, out_base=0) -> ffi.GrgHipGatherArgs:
+              sched: _LevelSchedule | None = None, sched_base: int = 0, mean_degree: float = 1.0,
+              out_base=0) -> ffi.GrgHipGatherArgs:
+        """Launch descriptor for rows [rows); ``sched`` adds the level's heavy rows."""
         a = ffi.GrgHipGatherArgs()
         a.dtype = self._dtype_code
         a.accumulate = 1 if accumulate else 0
         a.row_begin, a.row_end = int(rows[0]), int(rows[1])
         a.k = int(k)
+        if sched is not None:
+            mean_degree = sched.mean_degree
+            a.heavy_threshold = sched.heavy_threshold
+            a.heavy_rows, a.num_heavy = sched_base + 4 * sched.heavy_offset, sched.num_heavy
+        else:
+            a.heavy_threshold, a.heavy_rows, a.num_heavy = -1, 0, 0
+        a.flags = 0
         lds = [ld_out] + [s[3] for s in segs] + [m[2] for m in maps]
-        a.cpl, a.kl, a.es = launch_shape(self._dtype_code, int(k), lds, avg_degree)
+        a.cpl, a.kl, a.es = launch_shape(self._dtype_code, int(k), lds, mean_degree)
         a.out, a.ld_out, a.out_base = int(out), int(ld_out), int(out_base)
         a.nseg = len(segs)
         for i, (ptr, idx, src, ld, base) in enumerate(segs):
@@ -517,11 +588,42 @@ class HipRuntime:
         if row_scale is not None:
             a.row_scale, a.scale_base = int(row_scale[0]), int(row_scale[1])
         a.col_vec = int(col_vec or 0)
-        if heavy is None:
-            a.heavy_threshold, a.heavy_rows, a.num_heavy = -1, 0, 0
-        else:
-            a.heavy_threshold, a.heavy_rows, a.num_heavy = int(heavy[0]), int(heavy[1]), int(heavy[2])
         return a
+
+    def _tune(self, a: ffi.GrgHipGatherArgs, key) -> None:
+        """Per-level autotuner: time every admissible (cpl, kl, es, XCD remap) launch
+        shape of this level launch on the device and keep the fastest (cached per
+        artifact, sweep configuration, level and k). Run at prepare time, never in
+        ``op()``: the timing launches write only the level's node rows."""
+        torch = _torch()
+        best = self._tune_cache.get(key)
+        if best is None and self.autotune and a.row_end > a.row_begin:
+            lds = [a.ld_out] + [a.seg[i].ld for i in range(a.nseg)] + [a.map[i].ld for i in range(a.nmap)]
+            cands = [(cpl, kl, es, fl)
+                     for cpl, kl in column_shapes(a.dtype, a.k, lds)
+                     for es in (1, 2, 4, 8, 16, 32, 64) if kl * es <= 64
+                     for fl in (0, ffi.FLAG_XCD_REMAP)]
+            root = self._root_stream
+            ptr = int(root.cuda_stream)
+            trial = ffi.GrgHipGatherArgs()
+            ctypes.pointer(trial)[0] = a
+            timings = []
+            with torch.cuda.device(self.device):
+                for cpl, kl, es, fl in cands:
+                    trial.cpl, trial.kl, trial.es, trial.flags = cpl, kl, es, fl
+                    self._lib.gather(trial, ptr)  # warm
+                    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+                    ev[0].record(root)
+                    for i in range(3):
+                        self._lib.gather(trial, ptr)
+                        ev[i + 1].record(root)
+                    ev[-1].synchronize()
+                    t = min(ev[i].elapsed_time(ev[i + 1]) for i in range(3))
+                    timings.append((t, cpl, kl, es, fl))
+            best = min(timings)[1:]
+            self._tune_cache[key] = best
+        if best is not None:
+            a.cpl, a.kl, a.es, a.flags = best
 
     def _build_steps(self, art: _HipArtifact, spec: _CudaMatmulSpec):
         """The prepared sweep: a list of ("batch", array, n) and ("stream", step, args, events) items."""
@@ -536,12 +638,7 @@ class HipRuntime:
         max_k = int(self._io0.shape[1])
         io0 = self._io0.data_ptr()
         io1 = self._io1.data_ptr() if self._io1 is not None else 0
-        H = int(self.layout.heavy_threshold)
-        heavy_base = D["heavy"].data_ptr()
-
-        def heavy_for(config, level):
-            o, n = art.heavy[(config, level)]
-            return (H, heavy_base + 4 * o, n)
+        sched_base = D["sched"].data_ptr()
 
         # init terms folded into the sweep (backend_init_mode, as reference grg/__init__.py:498-502)
         maps_init = []
@@ -579,24 +676,27 @@ class HipRuntime:
                 if l > 0:
                     segs.append((D["up_ptr"].data_ptr(), D["up_idx"].data_ptr(), V, ldv, 0))
                 else:
-                    s2i = D["s2i"].data_ptr() if spec.by_individual else 0
-                    maps.insert(0, (s2i, io0, max_k, 0, st.num_samples))
+                    in_map = D["in_map_ind" if spec.by_individual else "in_map"].data_ptr()
+                    maps.insert(0, (in_map, io0, max_k, 0, st.num_samples))
             else:
                 segs.append((D["dn_ptr"].data_ptr(), D["dn_idx"].data_ptr(), V, ldv, 0))
                 segs.append((D["selT_ptr"].data_ptr(), D["selT_idx"].data_ptr(), io0, max_k, 0))
                 if spec.use_miss:
                     segs.append((D["missT_ptr"].data_ptr(), D["missT_idx"].data_ptr(), io1, max_k, 0))
-            heavy = heavy_for(cfg, l) if segs else None
-            batch.append(self._args(rows=(lo, hi), k=k, out=V, ld_out=ldv, segs=segs, maps=maps,
-                                    row_scale=row_scale_init, col_vec=col_vec_init, heavy=heavy,
-                                    avg_degree=art.level_avg_degree.get((cfg, l), 1.0) if segs else 1.0))
+            sched = art.sched[(cfg, l)] if segs else None
+            level_args = self._args(rows=(lo, hi), k=k, out=V, ld_out=ldv, segs=segs, maps=maps,
+                                    row_scale=row_scale_init, col_vec=col_vec_init, sched=sched, sched_base=sched_base)
+            if segs:
+                self._tune(level_args, (art.path, cfg, l, k))
+            batch.append(level_args)
             for i, step in enumerate(streams[l]):
                 flush()
                 slot_ptr = self._slot_indptr[step.slot].data_ptr()
                 slot_idx = self._slot_indices[step.slot].data_ptr()
+                mean = step.nnz / max(1, step.row_hi - step.row_lo)
                 args = self._args(rows=(step.row_lo, step.row_hi), k=k, out=V, ld_out=ldv,
                                   segs=[(slot_ptr, slot_idx, V + step.src_offset * ldv * w, ldv, step.row_lo)],
-                                  accumulate=True, avg_degree=step.nnz / max(1, step.row_hi - step.row_lo))
+                                  accumulate=True, mean_degree=mean)
                 items.append(("stream", step, args, events[l][i], (l, i)))
         # write-out
         if spec.emit_all_nodes:
@@ -631,10 +731,10 @@ class HipRuntime:
             if spec.by_individual:
                 batch.append(self._args(rows=(0, st.num_individuals), k=k, out=io0, ld_out=max_k,
                                         segs=[(D["ind_ptr"].data_ptr(), D["ind_idx"].data_ptr(), V, ldv, 0)],
-                                        row_scale=rs, col_vec=cv, avg_degree=float(st.ploidy)))
+                                        row_scale=rs, col_vec=cv, mean_degree=float(st.ploidy)))
             else:
                 batch.append(self._args(rows=(0, st.num_samples), k=k, out=io0, ld_out=max_k,
-                                        maps=[(0, V, ldv, 0, st.num_samples)], row_scale=rs, col_vec=cv))
+                                        maps=[(D["out_map"].data_ptr(), V, ldv, 0, st.num_samples)], row_scale=rs, col_vec=cv))
         flush()
         return items
 

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 1
+#define GRG_HIP_ABI_VERSION 3
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -46,6 +46,10 @@ enum GrgHipDtype {
 
 #define GRG_HIP_MAX_SEGS 3
 #define GRG_HIP_MAX_MAPS 2
+
+/* Workgroups are dispatched round-robin over the 8 XCDs; remap them so each
+ * XCD (and its L2) works on one contiguous eighth of the rows. */
+#define GRG_HIP_FLAG_XCD_REMAP 2
 
 /* sum_{j in [ptr[row - ptr_base], ptr[row - ptr_base + 1])} src[idx[j] * ld + c] */
 typedef struct GrgHipSeg {
@@ -73,7 +77,8 @@ typedef struct GrgHipGatherArgs {
   int32_t k;            /* columns */
   int32_t cpl;          /* columns per lane: 1, 2 or 4; k, every ld and every base must be multiples */
   int32_t kl;           /* lanes across one column chunk (power of two, kl * cpl columns per chunk) */
-  int32_t es;           /* lanes splitting one row's entries (power of two, kl * es <= 64) */
+  int32_t es;           /* lanes interleaving over one row's entries (power of two, kl * es <= 64) */
+  int32_t flags;        /* GRG_HIP_FLAG_* launch options */
   void* out;            /* out row r lives at out + (r - out_base) * ld_out */
   int64_t ld_out;
   int64_t out_base;
@@ -84,10 +89,10 @@ typedef struct GrgHipGatherArgs {
   const void* row_scale;  /* indexed by row - scale_base */
   int64_t scale_base;
   const void* col_vec;    /* indexed by column */
-  /* Rows whose total segment degree exceeds heavy_threshold are skipped by the
-   * sub-wave path and produced instead by one workgroup each, taken from
-   * heavy_rows[0..num_heavy) (global row ids; the caller lists exactly those
-   * rows). heavy_threshold < 0 disables the split. */
+  /* Rows whose total segment degree exceeds heavy_threshold (when >= 0) are
+   * skipped by the per-row path and produced instead by one 256-thread
+   * workgroup each, taken from heavy_rows[0..num_heavy) (global row ids; the
+   * caller lists exactly those rows). */
   int64_t heavy_threshold;
   const int32_t* heavy_rows;
   int64_t num_heavy;
