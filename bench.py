@@ -1,14 +1,22 @@
 """GRG matmul benchmark: edge-updates/s of one up + down sweep pair on one MI355X.
 
 A step is one up sweep (samples -> mutations) followed by one down sweep
-(mutations -> samples) over one batch of k synthetic N(0,1) columns, through
-the drop-in op surface (``grg.prepare_matmul_cuda``), with inputs resident in
-HBM. Each edge-update is one (edge, column) add, so a step performs 2 * E * k.
-The workload is a seeded synthetic GRG generated in-process and written as a
-``.grg_spmv`` artifact (no datasets are downloaded).
+(mutations -> samples, with the missing-data input when the workload has
+missingness) over one batch of k synthetic N(0,1) columns, through the drop-in
+op surface (``grg.prepare_matmul_cuda``), with inputs resident in HBM. Each
+edge-update is one (edge, column) add, so a step performs 2 * E * k.
 
-Prints ONE JSON line (rank 0). ``--gpus`` above 1 is rejected: this build runs
-on one GPU.
+The default workload is the 500,000-sample x 5,000,000-mutation synthetic GRG
+(5 % missing genotypes, coalescence counts kept, k = 16), in FP64 and in the
+FP32-compensated form; it is generated in-process from a fixed seed and
+written as a ``.grg_spmv`` artifact (no datasets are downloaded).
+
+Timing: HIP events recorded on the stream the sweep kernels are launched on
+(``HipRuntime.sweep_timer``), bracketing all launches of one sweep; the
+roofline figure divides the sweep's algorithmic bytes by that time.
+
+Prints ONE JSON line. ``--gpus`` above 1 is rejected: this build runs on one
+GPU.
 """
 
 from __future__ import annotations
@@ -28,81 +36,123 @@ for _p in (ROOT / "grg-spmv_amd", ROOT):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+METRIC = "GRG matmul edge-updates/s (k=16, up+down), % HBM roofline; FP32c vs FP64"
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(stats: dict, k: int, w: int, direction: str) -> int:
-    """Compulsory traffic of one sweep: structure once, each node value written and read once,
-    endpoints read/written once, selector once."""
-    E, N, S, M, L, sel = stats["edges"], stats["nodes"], stats["samples"], stats["mutations"], stats["levels"], stats["sel_mut_nnz"]
+def algorithmic_bytes(stats: dict, k: int, w: int, direction: str, use_miss: bool) -> int:
+    """Compulsory HBM traffic of one sweep: the structure once (int32 columns and
+    row pointers of the merged per-level CSR), every node value written once and
+    read once, the endpoint input read and output written once, the selector once
+    (and the missingness selector plus its M x k input when the down sweep takes
+    one)."""
+    E, N, S, M, L = stats["edges"], stats["nodes"], stats["samples"], stats["mutations"], stats["levels"]
     R = N + L
     n_in, n_out = (S, M) if direction == "up" else (M, S)
-    return int(4 * E + 4 * R + 2 * N * k * w + (n_in + n_out) * k * w + 4 * (sel + M + 1))
+    b = 4 * E + 4 * R + 2 * N * k * w + (n_in + n_out) * k * w + 4 * (stats["sel_mut_nnz"] + M + 1)
+    if use_miss:
+        b += 4 * (stats["sel_miss_nnz"] + M + 1) + M * k * w
+    return int(b)
 
 
-def run_gpu(artifact: Path, stats: dict, *, k: int, dtype, steps: int, warmup: int):
+def cpu_threads() -> int:
+    """Threads for the CPU baseline: the cores this process may use, capped by OMP_NUM_THREADS."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss: bool, calibrate_mib: int = 0):
     import torch
 
     from pygrgl_spmv.backends.hip import HipPlanPair, HipRuntime, plan_hip_layout
     from pygrgl_spmv.grg import RuntimeRequirements
 
-    req = RuntimeRequirements(max_k_up=k, max_k_down=k, need_down_miss_input=False, need_up_miss_output=False,
-                              need_init_vector=False, need_init_matrix=False, need_init_xtx=False)
+    req = RuntimeRequirements(max_k_up=k, max_k_down=k, need_down_miss_input=use_miss, need_up_miss_output=False,
+                              need_init_vector=False, need_init_matrix=False, need_init_xtx=use_miss)
     layout = plan_hip_layout(artifacts=[artifact], pair=HipPlanPair.default(), dtype=dtype, requirements=req,
                              vram_budget_bytes=0, ring_buffer_size=0, device=0, stream=0)
+    t0 = time.perf_counter()
     with HipRuntime(layout) as rt:
         (grg,) = rt.grgs
         up = grg.prepare_matmul_cuda(direction="up", k=k)
-        down = grg.prepare_matmul_cuda(direction="down", k=k)
+        down = grg.prepare_matmul_cuda(direction="down", k=k, use_miss=use_miss)
+        log(f"{np.dtype(dtype).name}: runtime entered and ops prepared in {time.perf_counter() - t0:.1f}s")
         rng = np.random.default_rng(2026)
-        x = torch.from_numpy(rng.standard_normal((k, grg.num_samples)).astype(dtype)).to(up.input.device)
-        up.input.copy_(x)
-        caller = rt.stream
+        dev = up.input.device
+        x = torch.from_numpy(rng.standard_normal((k, grg.num_samples)).astype(dtype)).to(dev)
+        y = torch.from_numpy(rng.standard_normal((k, grg.num_mutations)).astype(dtype)).to(dev)
+        miss = torch.from_numpy(rng.standard_normal((k, grg.num_mutations)).astype(dtype)).to(dev) if use_miss else None
+
+        def load_inputs(which):
+            # up's input and down's output share the arena (and down's input is up's output):
+            # inputs are restored outside the timed events
+            if which == "up":
+                up.input.copy_(x)
+            else:
+                down.input.copy_(y)
+                if miss is not None:
+                    down.miss_input.copy_(miss)
+
         for _ in range(warmup):
+            load_inputs("up")
             up()
+            load_inputs("down")
             down()
-            up.input.copy_(x)
-        torch.cuda.synchronize()
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
         torch.cuda.synchronize()
         wall0 = time.perf_counter()
-        for a, b, c in ev:
-            # down's output aliases up's input (shared arena); restore the batch outside the timed events
-            up.input.copy_(x)
-            a.record(caller)
-            up()
-            b.record(caller)
-            down()
-            c.record(caller)
+        with rt.sweep_timer() as ev:
+            for _ in range(steps):
+                load_inputs("up")
+                up()
+                load_inputs("down")
+                down()
         torch.cuda.synchronize()
         wall = time.perf_counter() - wall0
-        t_up = sum(a.elapsed_time(b) for a, b, _ in ev) / steps / 1e3
-        t_dn = sum(b.elapsed_time(c) for _, b, c in ev) / steps / 1e3
+        t_up = sum(a.elapsed_time(b) for a, b in ev[0::2]) / steps / 1e3
+        t_dn = sum(a.elapsed_time(b) for a, b in ev[1::2]) / steps / 1e3
         # one clean pass for the accuracy report
-        up.input.copy_(x)
+        load_inputs("up")
         up()
         up_out = up.output.detach().cpu().numpy().astype(np.float64)
+        load_inputs("down")
         down()
         down_out = down.output.detach().cpu().numpy().astype(np.float64)
+        if calibrate_mib > 0:
+            # a plain device copy of known size, for scaling HBM byte counters
+            n = calibrate_mib * (1 << 20) // 8
+            a = torch.empty(n, dtype=torch.float64, device=dev).fill_(1.0)
+            b = torch.empty_like(a)
+            torch.cuda.synchronize()
+            b.copy_(a)
+            torch.cuda.synchronize()
+            log(f"calibration copy: {calibrate_mib} MiB read + {calibrate_mib} MiB written")
+            del a, b
     return {"t_up": t_up, "t_down": t_dn, "wall_per_step": wall / steps, "up_out": up_out, "down_out": down_out}
 
 
-def cpu_baseline(artifact: Path, stats: dict, *, k: int, budget_s: float = 20.0):
+def cpu_baseline(artifact: Path, stats: dict, *, k: int, use_miss: bool, budget_s: float = 20.0):
     """The reference's MklRuntime sweep loop (restated in oracle/mkl_baseline.py) on this host's cores."""
     from oracle import grg_oracle as O
     from oracle.mkl_baseline import MklSweep
 
     g = O.load_artifact(artifact)
-    sweep = MklSweep(g, max_k=k)
+    sweep = MklSweep(g, max_k=k, n_threads=cpu_threads())
     try:
         rng = np.random.default_rng(2026)
         x = rng.standard_normal((g.num_samples, k))
         y = rng.standard_normal((g.num_mutations, k))
+        miss = rng.standard_normal((g.num_mutations, k)) if use_miss else None
         v_up = np.zeros((g.num_nodes, k))
         v_dn = np.zeros((g.num_nodes, k))
+        seed_dn = g.sel_mut.T @ y
+        if miss is not None:
+            seed_dn = seed_dn + g.sel_miss.T @ miss
 
         def one():
             v_up[:] = 0.0
@@ -110,8 +160,7 @@ def cpu_baseline(artifact: Path, stats: dict, *, k: int, budget_s: float = 20.0)
             t0 = time.perf_counter()
             sweep.sweep(v_up, "up")
             t1 = time.perf_counter()
-            v_dn[:] = 0.0
-            v_dn[:] = g.sel_mut.T @ y
+            v_dn[:] = seed_dn
             t2 = time.perf_counter()
             sweep.sweep(v_dn, "down")
             t3 = time.perf_counter()
@@ -130,9 +179,24 @@ def cpu_baseline(artifact: Path, stats: dict, *, k: int, budget_s: float = 20.0)
         "unit": "edge-updates/s",
         "cores": int(sweep.n_threads),
         "kind": "port",
-        "sample": f"MKL per-block sweep (up + down, k={k}, mean of {len(times)} steps after 1 warmup) on the full workload; library {sweep.path}",
+        "sample": f"MKL per-block level sweep (mkl_sparse_d_mm, up N/CSR + down T on the same handles, k={k}) over the "
+                  f"whole workload, mean of {len(times)} up+down steps after 1 warmup; seeding and selector gathers "
+                  f"outside the timed sweep; {sweep.n_threads} threads; library {sweep.path}",
         "ms_per_step": per_step * 1e3,
+        "cpu_count": os.cpu_count(),
     }
+
+
+def load_traffic(workload: str, k: int):
+    """HBM bytes per sweep measured by rocprofv3 PMC runs of this benchmark (FETCH_SIZE and
+    WRITE_SIZE over the sweep's dispatches, scaled by a plain copy), if recorded."""
+    p = ROOT / "profiles" / f"row_gather_{workload}_k{k}_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text())
+    except (OSError, ValueError):
+        return None
 
 
 def main() -> int:
@@ -140,16 +204,20 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c3")
     ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--artifact-dir", default=os.environ.get("GRG_BENCH_DIR", "/tmp/grg_bench"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="FP32c only (profiling runs)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after the timed region, run one device copy of this many MiB (for PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
         return 2
 
+    from pygrgl_spmv.grg.artifact import scan_grg_spmv
     from pygrgl_spmv.synth import WORKLOADS, write_artifact
 
     cfg = WORKLOADS[args.workload]
@@ -158,62 +226,85 @@ def main() -> int:
     artifact = art_dir / f"{args.workload}-seed{cfg.seed}.grg_spmv"
     t0 = time.perf_counter()
     if not artifact.exists():
+        log(f"generating workload {args.workload} into {artifact}")
         write_artifact(cfg, artifact)
-    from pygrgl_spmv.grg.artifact import scan_grg_spmv
-
     scan = scan_grg_spmv(artifact)
     stats = {"samples": scan.num_samples, "mutations": scan.num_mutations, "nodes": scan.num_nodes,
-             "edges": scan.num_edges, "levels": scan.num_levels, "sel_mut_nnz": scan.selector_mut_nnz}
-    log(f"workload {args.workload}: {stats} ready in {time.perf_counter() - t0:.1f}s")
+             "edges": scan.num_edges, "levels": scan.num_levels, "sel_mut_nnz": scan.selector_mut_nnz,
+             "sel_miss_nnz": scan.selector_miss_nnz}
+    use_miss = bool(scan.has_missing_data) and stats["sel_miss_nnz"] > 0
+    log(f"workload {args.workload}: {stats} ready in {time.perf_counter() - t0:.1f}s; down sweep uses miss input: {use_miss}")
 
     k = int(args.k)
-    f64 = run_gpu(artifact, stats, k=k, dtype=np.float64, steps=args.steps, warmup=args.warmup)
     eu = 2 * stats["edges"] * k
-    t64 = f64["t_up"] + f64["t_down"]
-    b64 = algorithmic_bytes(stats, k, 8, "up") + algorithmic_bytes(stats, k, 8, "down")
-    log(f"fp64: up {f64['t_up']*1e3:.3f} ms, down {f64['t_down']*1e3:.3f} ms, {eu / t64 / 1e9:.2f} G edge-updates/s, {b64 / t64 / 1e9:.0f} GB/s")
-    variants = {"fp64": {"value": eu / t64, "ms_per_step": t64 * 1e3, "ms_up": f64["t_up"] * 1e3, "ms_down": f64["t_down"] * 1e3,
-                         "algorithmic_GBs": b64 / t64 / 1e9}}
+    variants = {}
+    f64 = None
+    if not args.no_fp64:
+        f64 = run_gpu(artifact, k=k, dtype=np.float64, steps=args.steps, warmup=args.warmup, use_miss=use_miss,
+                      calibrate_mib=args.calibrate_copy_mib)
+        t64 = f64["t_up"] + f64["t_down"]
+        b64 = algorithmic_bytes(stats, k, 8, "up", False) + algorithmic_bytes(stats, k, 8, "down", use_miss)
+        log(f"fp64: up {f64['t_up']*1e3:.3f} ms, down {f64['t_down']*1e3:.3f} ms, {eu / t64 / 1e9:.2f} G edge-updates/s, "
+            f"{b64 / t64 / 1e9:.0f} GB/s algorithmic")
+        variants["fp64"] = {"value": eu / t64, "ms_per_step": t64 * 1e3, "ms_up": f64["t_up"] * 1e3,
+                            "ms_down": f64["t_down"] * 1e3, "algorithmic_bytes": b64, "algorithmic_GBs": b64 / t64 / 1e9,
+                            "wall_ms_per_step": f64["wall_per_step"] * 1e3}
     if not args.no_fp32c:
-        f32 = run_gpu(artifact, stats, k=k, dtype=np.float32, steps=args.steps, warmup=args.warmup)
+        f32 = run_gpu(artifact, k=k, dtype=np.float32, steps=args.steps, warmup=args.warmup, use_miss=use_miss,
+                      calibrate_mib=args.calibrate_copy_mib if args.no_fp64 else 0)
         t32 = f32["t_up"] + f32["t_down"]
-        b32 = algorithmic_bytes(stats, k, 4, "up") + algorithmic_bytes(stats, k, 4, "down")
+        b32 = algorithmic_bytes(stats, k, 4, "up", False) + algorithmic_bytes(stats, k, 4, "down", use_miss)
+        v = {"value": eu / t32, "ms_per_step": t32 * 1e3, "ms_up": f32["t_up"] * 1e3, "ms_down": f32["t_down"] * 1e3,
+             "algorithmic_bytes": b32, "algorithmic_GBs": b32 / t32 / 1e9, "wall_ms_per_step": f32["wall_per_step"] * 1e3}
+        msg = f"fp32c: up {f32['t_up']*1e3:.3f} ms, down {f32['t_down']*1e3:.3f} ms, {eu / t32 / 1e9:.2f} G edge-updates/s"
+        if f64 is not None:
+            def col_err(a, b):
+                # per-column relative error (max-norm over rows) against the FP64 result
+                return np.abs(a - b).max(axis=1) / np.maximum(np.abs(b).max(axis=1), 1e-300)
 
-        def rel(a, b):
-            return float((np.abs(a - b).max(axis=1) / np.maximum(np.abs(b).max(axis=1), 1e-300)).max())
-
-        err = max(rel(f32["up_out"], f64["up_out"]), rel(f32["down_out"], f64["down_out"]))
-        log(f"fp32c: {eu / t32 / 1e9:.2f} G edge-updates/s, {b32 / t32 / 1e9:.0f} GB/s, rel err vs fp64 {err:.2e}")
-        variants["fp32c"] = {"value": eu / t32, "ms_per_step": t32 * 1e3, "ms_up": f32["t_up"] * 1e3, "ms_down": f32["t_down"] * 1e3,
-                             "algorithmic_GBs": b32 / t32 / 1e9, "speedup_vs_fp64": t64 / t32, "max_rel_err_vs_fp64": err}
+            e_up = col_err(f32["up_out"], f64["up_out"])
+            e_dn = col_err(f32["down_out"], f64["down_out"])
+            v.update({"speedup_vs_fp64": (f64["t_up"] + f64["t_down"]) / t32,
+                      "max_rel_err_vs_fp64": float(max(e_up.max(), e_dn.max())),
+                      "per_column_rel_err_up": [float(e) for e in e_up],
+                      "per_column_rel_err_down": [float(e) for e in e_dn]})
+            msg += f", {v['speedup_vs_fp64']:.2f}x fp64, max per-column rel err {v['max_rel_err_vs_fp64']:.2e}"
+        log(msg)
+        variants["fp32c"] = v
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(artifact, stats, k=k)
+            cpu = cpu_baseline(artifact, stats, k=k, use_miss=use_miss)
             log(f"cpu baseline: {cpu['value'] / 1e9:.3f} G edge-updates/s on {cpu['cores']} threads")
         except Exception as exc:  # reported, not fatal
-            cpu = {"value": None, "unit": "edge-updates/s", "cores": os.cpu_count(), "kind": "port", "sample": f"unavailable: {exc}"}
+            cpu = {"value": None, "unit": "edge-updates/s", "cores": cpu_threads(), "kind": "port", "sample": f"unavailable: {exc}"}
+    head = variants.get("fp64") or variants["fp32c"]
+    traffic = load_traffic(args.workload, k)
     out = {
-        "metric": "GRG matmul edge-updates/s (k=16, up+down), % HBM roofline; FP32c vs FP64",
-        "value": variants["fp64"]["value"],
+        "metric": METRIC,
+        "value": head["value"],
         "unit": "edge-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": variants["fp64"]["ms_per_step"],
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if "fp64" in variants else "f32c",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: synthetic GRG {stats['samples']} samples x {stats['mutations']} mutations "
-                               f"({stats['nodes']} nodes, {stats['edges']} edges, {stats['levels']} levels), k={k}, up+down",
+                               f"({stats['nodes']} nodes, {stats['edges']} edges, {stats['levels']} levels), k={k}, up + down"
+                               + (" with missing-data input" if use_miss else ""),
                    "k": k, **stats},
-        "roofline": {"bound": "hbm", "achieved": b64 / t64 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": b64 / t64 / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "roofline": {"bound": "hbm", "achieved": head["algorithmic_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": head["algorithmic_GBs"] / HBM_PEAK_GBS,
+                     "traffic": None if traffic is None else traffic.get("bytes_per_step")},
         "variants": variants,
         "cpu_baseline": cpu,
     }
+    if traffic is not None:
+        out["roofline"]["traffic_source"] = traffic.get("source")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -262,6 +262,7 @@ class HipRuntime:
         self._artifacts = ()
         self._grgs = ()
         self._tune_cache = {}
+        self._sweep_timer = None
         self._entered = False
         self._active_call = False
 
@@ -347,6 +348,19 @@ class HipRuntime:
                 with torch.cuda.stream(self._root_stream):
                     self._root_to_caller_event.record(self._root_stream)
                 self._caller_stream.wait_event(self._root_to_caller_event)
+
+    @contextmanager
+    def sweep_timer(self):
+        """Record a pair of timing events on the launch stream around every op run
+        inside the block; yields the list of (start, end) events (read them after a
+        synchronize). Used to time the kernels on the stream they run on."""
+        if self._sweep_timer is not None:
+            raise RuntimeError("sweep_timer() is already active")
+        self._sweep_timer = []
+        try:
+            yield self._sweep_timer
+        finally:
+            self._sweep_timer = None
 
     def _prepare_matmul_cuda(self, grg: BoundGRG, spec: _CudaMatmulSpec) -> _HipPreparedMatmul:
         if spec.direction == Direction.UP and self.layout.pair.plan_up is None:
@@ -746,6 +760,10 @@ class HipRuntime:
             root = self._root_stream
             root_ptr = int(root.cuda_stream)
             has_stream = any(it[0] == "stream" for it in items)
+            timer = self._sweep_timer
+            if timer is not None:
+                t_start, t_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                t_start.record(root)
             if has_stream:
                 with torch.cuda.stream(root):
                     self._launch_event.record(root)
@@ -771,6 +789,9 @@ class HipRuntime:
                 self._lib.gather(args, root_ptr)
                 with torch.cuda.stream(root):
                     compute_done.record(root)
+            if timer is not None:
+                t_end.record(root)
+                timer.append((t_start, t_end))
 
 
 __all__ = ["HipRuntime", "launch_shape"]
