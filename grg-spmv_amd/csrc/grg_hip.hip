@@ -15,22 +15,33 @@
 // lanes, one row per group; kl lanes cover a chunk of kl * cpl columns with
 // cpl-wide vector loads, es lanes interleave over the row's entries (so the
 // group's lanes read adjacent entries, which the locality order of the node
-// ids keeps close in memory). Each lane works in rounds of kUnroll
+// ids keeps close in memory). Each lane works in rounds of U (4, 8 or 16)
 // predicated gathers: all index loads of the round, then all row loads, then
-// the adds, so a row of up to kUnroll * es entries costs one round trip.
+// the adds, so a row of up to U * es entries costs one round trip; loads in
+// flight stay in the storage type (U * cpl values per lane).
 // The es partial sums are combined with cross-lane xor shuffles.
 //
 // Heavy path (same launch): rows of high degree get one 256-thread
 // workgroup each (256 / kl edge lanes), reduced with shuffles plus one LDS
 // pass.
 //
+// Split rows: a row too long for one workgroup (for example a missingness
+// pool node referenced by every mutation) is cut into fixed-size chunks of
+// its concatenated segment entries; a partials launch (flag
+// GRG_HIP_FLAG_PARTIALS) gives each chunk a workgroup that writes one FP64
+// partial row, and the level launch's heavy path then sums the row's
+// partials instead of its entries. Both launches are ordinary stream-ordered
+// kernels, so the pair stays deterministic and graph-capturable.
+//
 // Optional XCD remap: workgroups are dispatched round-robin over the 8 XCDs,
 // so consecutive row blocks would land on different L2s; the remap gives each
 // XCD one contiguous eighth of the rows, keeping neighbouring rows (which
 // share children in the locality order) on one L2.
 //
-// FP32 values use compensated (TwoSum) accumulation per row; FP64 values use
-// plain FP64 adds.
+// FP64 values are summed in FP64. FP32 values are summed as two-float
+// compensated pairs (TwoSum), with loads in flight held as FP32 so the FP32
+// form keeps half the registers per gathered row; split-row partials are
+// kept in FP64 for both.
 
 #include <hip/hip_runtime.h>
 
@@ -50,56 +61,14 @@ int fail(const std::string& msg, int code = 1) {
 
 constexpr int kBlock = 256;  // threads per workgroup
 constexpr int kWavesPerBlock = kBlock / 64;
-constexpr int kUnroll = 8;   // gathers in flight per lane and round
-
-template <typename T>
-struct PlainAcc {
-  T s;
-  __device__ __forceinline__ void zero() { s = T(0); }
-  __device__ __forceinline__ void add(T v) { s += v; }
-  __device__ __forceinline__ void merge_xor(int off) { s += __shfl_xor(s, off, 64); }
-  __device__ __forceinline__ void merge(const PlainAcc& o) { s += o.s; }
-  __device__ __forceinline__ T value() const { return s; }
-};
-
-// Two-float accumulator: hi carries the running FP32 sum, lo the exact
-// rounding errors of every addition (Knuth TwoSum), folded in at the end.
-struct CompAcc {
-  float hi, lo;
-  __device__ __forceinline__ void zero() { hi = 0.f; lo = 0.f; }
-  __device__ __forceinline__ void add(float v) {
-    float s = hi + v;
-    float bp = s - hi;
-    float err = (hi - (s - bp)) + (v - bp);
-    hi = s;
-    lo += err;
-  }
-  __device__ __forceinline__ void merge2(float h2, float l2) {
-    float s = hi + h2;
-    float bp = s - hi;
-    float err = (hi - (s - bp)) + (h2 - bp);
-    hi = s;
-    lo += l2 + err;
-  }
-  __device__ __forceinline__ void merge_xor(int off) {
-    float h2 = __shfl_xor(hi, off, 64);
-    float l2 = __shfl_xor(lo, off, 64);
-    merge2(h2, l2);
-  }
-  __device__ __forceinline__ void merge(const CompAcc& o) { merge2(o.hi, o.lo); }
-  __device__ __forceinline__ float value() const { return hi + lo; }
-};
-
-template <typename T, bool COMP>
-struct AccOf { using type = PlainAcc<T>; };
-template <>
-struct AccOf<float, true> { using type = CompAcc; };
 
 template <typename T, int N>
 struct VecT { typedef T type __attribute__((ext_vector_type(N))); };
 
+// Load cpl consecutive values (one vector load), kept in their storage type
+// so that loads in flight hold as few registers as possible.
 template <typename T, int CPL>
-__device__ __forceinline__ void load_cols(const T* p, T (&v)[CPL]) {
+__device__ __forceinline__ void load_raw(const T* p, T (&v)[CPL]) {
   if constexpr (CPL == 1) {
     v[0] = *p;
   } else {
@@ -110,7 +79,7 @@ __device__ __forceinline__ void load_cols(const T* p, T (&v)[CPL]) {
 }
 
 template <typename T, int CPL>
-__device__ __forceinline__ void store_cols(T* p, const T (&v)[CPL]) {
+__device__ __forceinline__ void store_raw(T* p, const T (&v)[CPL]) {
   if constexpr (CPL == 1) {
     *p = v[0];
   } else {
@@ -121,30 +90,71 @@ __device__ __forceinline__ void store_cols(T* p, const T (&v)[CPL]) {
   }
 }
 
+// FP64 values: plain FP64 sum.
+struct SumF64 {
+  double s;
+  __device__ __forceinline__ void zero() { s = 0.0; }
+  __device__ __forceinline__ void add(double v) { s += v; }
+  __device__ __forceinline__ void add_wide(double v) { s += v; }
+  __device__ __forceinline__ void merge(const SumF64& o) { s += o.s; }
+  __device__ __forceinline__ void merge_xor(int off) { s += __shfl_xor(s, off, 64); }
+  __device__ __forceinline__ double wide() const { return s; }
+  __device__ __forceinline__ double value() const { return s; }
+};
+
+// FP32 values: two-float compensated sum. hi carries the running FP32 sum,
+// lo the exact rounding error of every addition (Knuth's TwoSum), so a row
+// sum is accurate to about FP32 epsilon squared before its single rounding
+// to FP32 at the store.
+struct SumF32C {
+  float hi, lo;
+  __device__ __forceinline__ void zero() { hi = 0.f; lo = 0.f; }
+  __device__ __forceinline__ void add2(float h2, float l2) {
+    const float s = hi + h2;
+    const float bp = s - hi;
+    const float err = (hi - (s - bp)) + (h2 - bp);
+    hi = s;
+    lo += l2 + err;
+  }
+  __device__ __forceinline__ void add(float v) { add2(v, 0.f); }
+  __device__ __forceinline__ void add_wide(double v) {  // an FP64 partial as an unevaluated float pair
+    const float h = (float)v;
+    add2(h, (float)(v - (double)h));
+  }
+  __device__ __forceinline__ void merge(const SumF32C& o) { add2(o.hi, o.lo); }
+  __device__ __forceinline__ void merge_xor(int off) { add2(__shfl_xor(hi, off, 64), __shfl_xor(lo, off, 64)); }
+  __device__ __forceinline__ double wide() const { return (double)hi + (double)lo; }
+  __device__ __forceinline__ float value() const { return hi + lo; }
+};
+
+template <typename T> struct SumOf;
+template <> struct SumOf<double> { using type = SumF64; };
+template <> struct SumOf<float> { using type = SumF32C; };
+
 // Sum src[idx[j]] for j = first, first + stride, ... < end into acc, in rounds
-// of kUnroll predicated gathers (absent entries add an exact zero).
-template <typename T, int CPL, typename Acc>
+// of U predicated gathers (absent entries add an exact zero).
+template <typename T, int CPL, int U, typename Acc>
 __device__ __forceinline__ void gather_entries(const int32_t* __restrict__ idx, const T* __restrict__ src, int64_t ld,
                                                int32_t first, int32_t end, int32_t stride, Acc (&acc)[CPL]) {
-  for (int32_t j = first; j < end; j += kUnroll * stride) {
-    int32_t ii[kUnroll];
+  for (int32_t j = first; j < end; j += U * stride) {
+    int32_t ii[U];
 #pragma unroll
-    for (int t = 0; t < kUnroll; ++t) {
+    for (int t = 0; t < U; ++t) {
       const int32_t jt = j + t * stride;
       ii[t] = jt < end ? idx[jt] : -1;
     }
-    T v[kUnroll][CPL];
+    T v[U][CPL];
 #pragma unroll
-    for (int t = 0; t < kUnroll; ++t) {
+    for (int t = 0; t < U; ++t) {
       if (ii[t] >= 0) {
-        load_cols<T, CPL>(src + (int64_t)ii[t] * ld, v[t]);
+        load_raw<T, CPL>(src + (int64_t)ii[t] * ld, v[t]);
       } else {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) v[t][c] = T(0);
       }
     }
 #pragma unroll
-    for (int t = 0; t < kUnroll; ++t)
+    for (int t = 0; t < U; ++t)
 #pragma unroll
       for (int c = 0; c < CPL; ++c) acc[c].add(v[t][c]);
   }
@@ -158,7 +168,7 @@ __device__ __forceinline__ void add_row_terms(const GrgHipGatherArgs& a, int64_t
     if (row >= mp.row_lo && row < mp.row_hi) {
       const int64_t r = mp.map ? (int64_t)mp.map[row - mp.row_lo] : (row - mp.row_lo);
       T v[CPL];
-      load_cols<T, CPL>(static_cast<const T*>(mp.src) + r * mp.ld + col, v);
+      load_raw<T, CPL>(static_cast<const T*>(mp.src) + r * mp.ld + col, v);
 #pragma unroll
       for (int c = 0; c < CPL; ++c) acc[c].add(v[c]);
     }
@@ -168,14 +178,40 @@ __device__ __forceinline__ void add_row_terms(const GrgHipGatherArgs& a, int64_t
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
       const T cv = a.col_vec ? static_cast<const T*>(a.col_vec)[col + c] : T(1);
-      acc[c].add(rs * cv);
+      acc[c].add_wide((double)rs * (double)cv);
     }
   }
   if (a.accumulate) {
     T v[CPL];
-    load_cols<T, CPL>(static_cast<const T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
+    load_raw<T, CPL>(static_cast<const T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
 #pragma unroll
     for (int c = 0; c < CPL; ++c) acc[c].add(v[c]);
+  }
+}
+
+// Workgroup reduction of the heavy / partials paths: lanes with the same
+// column lane (threadIdx.x % KL) are summed; the result is valid in wave 0,
+// lanes 0..KL-1.
+template <int CPL, typename Acc>
+__device__ __forceinline__ void block_reduce(Acc (&acc)[CPL], int KL, Acc (&partial)[kWavesPerBlock][64]) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  for (int off = KL; off < 64; off <<= 1) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) acc[c].merge_xor(off);
+  }
+  if (lane < KL) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) partial[wave][lane * CPL + c] = acc[c];
+  }
+  __syncthreads();
+  if (wave == 0 && lane < KL) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      Acc t = partial[0][lane * CPL + c];
+      for (int w = 1; w < kWavesPerBlock; ++w) t.merge(partial[w][lane * CPL + c]);
+      acc[c] = t;
+    }
   }
 }
 
@@ -183,14 +219,14 @@ template <typename T, int CPL, typename Acc>
 __device__ __forceinline__ void store_row(const GrgHipGatherArgs& a, int64_t row, int col, const Acc (&acc)[CPL]) {
   T v[CPL];
 #pragma unroll
-  for (int c = 0; c < CPL; ++c) v[c] = acc[c].value();
-  store_cols<T, CPL>(static_cast<T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
+  for (int c = 0; c < CPL; ++c) v[c] = (T)acc[c].value();
+  store_raw<T, CPL>(static_cast<T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
 }
 
-template <typename T, int CPL, bool COMP>
+template <typename T, int CPL, int U>
 __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherArgs a, int64_t n_light_blocks,
                                                              int g_shift, int kl_shift) {
-  using Acc = typename AccOf<T, COMP>::type;
+  using Acc = typename SumOf<T>::type;
   __shared__ Acc partial[kWavesPerBlock][64];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -198,6 +234,40 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
   Acc acc[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) acc[c].zero();
+
+  if (a.flags & GRG_HIP_FLAG_PARTIALS) {
+    // ---- partials launch: workgroup i sums chunk j of split heavy row h ----
+    const int64_t item = blockIdx.x;
+    const int32_t h = a.split_row[item];
+    const int64_t row = a.heavy_rows[h];
+    const int64_t lo = (item - a.split_ptr[h]) * a.split_chunk;
+    const int64_t hi = lo + a.split_chunk;
+    const int cl = threadIdx.x & (KL - 1);
+    const int e = threadIdx.x >> kl_shift;
+    const int ES = kBlock >> kl_shift;
+    const int col = ((int)blockIdx.y * KL + cl) * CPL;
+    if (col < a.k) {
+      int64_t g = 0;  // offset of segment s in the concatenated entry list
+      for (int s = 0; s < a.nseg; ++s) {
+        const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
+        const int32_t p0 = ptr[0], p1 = ptr[1];
+        const int64_t n = p1 - p0;
+        const int64_t b = lo > g ? lo - g : 0;
+        const int64_t t = hi - g < n ? hi - g : n;
+        if (b < t)
+          gather_entries<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld,
+                                    p0 + (int32_t)b + e, p0 + (int32_t)t, ES, acc);
+        g += n;
+      }
+    }
+    block_reduce<CPL>(acc, KL, partial);
+    if (wave == 0 && lane < KL && col < a.k) {
+      double* dst = static_cast<double*>(a.partials) + item * a.k + col;
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) dst[c] = acc[c].wide();
+    }
+    return;
+  }
 
   if ((int64_t)blockIdx.x < n_light_blocks) {
     // ---- light path: one row per G-lane group ----
@@ -227,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
         live = false;  // produced by the heavy path
       } else {
         for (int s = 0; s < a.nseg; ++s)
-          gather_entries<T, CPL>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, p0[s] + e, p1[s], ES, acc);
+          gather_entries<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, p0[s] + e, p1[s], ES, acc);
         if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
       }
     }
@@ -240,38 +310,33 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
   }
 
   // ---- heavy path: the whole workgroup on one row ----
-  const int64_t row = a.heavy_rows[(int64_t)blockIdx.x - n_light_blocks];
+  const int64_t h = (int64_t)blockIdx.x - n_light_blocks;
+  const int64_t row = a.heavy_rows[h];
   const int cl = threadIdx.x & (KL - 1);
   const int e = threadIdx.x >> kl_shift;
   const int ES = kBlock >> kl_shift;
   const int col = ((int)blockIdx.y * KL + cl) * CPL;
   const bool live = col < a.k;
   if (live) {
-    for (int s = 0; s < a.nseg; ++s) {
-      const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
-      gather_entries<T, CPL>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, ptr[0] + e, ptr[1], ES, acc);
+    const int32_t it0 = a.split_ptr ? a.split_ptr[h] : 0;
+    const int32_t it1 = a.split_ptr ? a.split_ptr[h + 1] : 0;
+    if (it1 > it0) {
+      // split row: sum the FP64 partial rows written by the partials launch
+      const double* part = static_cast<const double*>(a.partials) + col;
+      for (int32_t it = it0 + e; it < it1; it += ES) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[c].add_wide(part[(int64_t)it * a.k + c]);
+      }
+    } else {
+      for (int s = 0; s < a.nseg; ++s) {
+        const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
+        gather_entries<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, ptr[0] + e, ptr[1], ES, acc);
+      }
     }
     if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
   }
-  for (int off = KL; off < 64; off <<= 1) {
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) acc[c].merge_xor(off);
-  }
-  // lanes 0..KL-1 of each wave hold the wave's partial for their column lane
-  if (lane < KL) {
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) partial[wave][lane * CPL + c] = acc[c];
-  }
-  __syncthreads();
-  if (wave == 0 && lane < KL && live) {
-    Acc tot[CPL];
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      tot[c] = partial[0][lane * CPL + c];
-      for (int w = 1; w < kWavesPerBlock; ++w) tot[c].merge(partial[w][lane * CPL + c]);
-    }
-    store_row<T, CPL>(a, row, col, tot);
-  }
+  block_reduce<CPL>(acc, KL, partial);
+  if (wave == 0 && lane < KL && live) store_row<T, CPL>(a, row, col, acc);
 }
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
@@ -293,7 +358,8 @@ int validate(const GrgHipGatherArgs& a) {
   if (a.cpl != 1 && a.cpl != 2 && a.cpl != 4) return fail("cpl must be 1, 2 or 4");
   if (!is_pow2(a.kl) || !is_pow2(a.es) || a.kl * a.es > 64) return fail("kl and es must be powers of two with kl * es <= 64");
   if (a.kl * a.cpl > 64) return fail("kl * cpl must be <= 64");
-  if (a.flags & ~GRG_HIP_FLAG_XCD_REMAP) return fail("unknown flags");
+  if (a.unroll != 0 && a.unroll != 4 && a.unroll != 8 && a.unroll != 16) return fail("unroll must be 0, 4, 8 or 16");
+  if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS)) return fail("unknown flags");
   if (a.k % a.cpl) return fail("k must be a multiple of cpl");
   if (!a.out) return fail("out is null");
   if (a.nseg < 0 || a.nseg > GRG_HIP_MAX_SEGS) return fail("nseg out of range");
@@ -310,36 +376,58 @@ int validate(const GrgHipGatherArgs& a) {
   }
   if (a.num_heavy < 0 || (a.num_heavy > 0 && !a.heavy_rows)) return fail("heavy rows missing");
   if (a.num_heavy > 0 && a.heavy_threshold < 0) return fail("heavy rows need heavy_threshold >= 0");
+  if (a.split_ptr && (!a.partials || a.num_heavy < 1)) return fail("split rows need partials and heavy rows");
+  if (a.flags & GRG_HIP_FLAG_PARTIALS) {
+    if (!a.split_ptr || !a.split_row || !a.partials) return fail("partials launch needs split_ptr, split_row and partials");
+    if (!a.heavy_rows || a.num_heavy < 1) return fail("partials launch needs the heavy rows");
+    if (a.split_items < 0) return fail("split_items < 0");
+    if (a.split_chunk < 1 || a.split_chunk > 0x40000000LL) return fail("split_chunk out of range");
+    if (reinterpret_cast<uintptr_t>(a.partials) % (a.cpl * 8)) return fail("partials not aligned to cpl doubles");
+  }
   return 0;
 }
 
-template <typename T, int CPL, bool COMP>
+template <typename T, int CPL, int U>
 void launch_typed(const GrgHipGatherArgs& a, dim3 grid, int64_t n_light, int g_shift, int kl_shift, hipStream_t s) {
-  hipLaunchKernelGGL((row_gather_kernel<T, CPL, COMP>), grid, dim3(kBlock), 0, s, a, n_light, g_shift, kl_shift);
+  hipLaunchKernelGGL((row_gather_kernel<T, CPL, U>), grid, dim3(kBlock), 0, s, a, n_light, g_shift, kl_shift);
+}
+
+template <typename T, int CPL>
+void launch_unrolled(const GrgHipGatherArgs& a, dim3 grid, int64_t n_light, int g_shift, int kl_shift, hipStream_t s) {
+  switch (a.unroll) {
+    case 4: launch_typed<T, CPL, 4>(a, grid, n_light, g_shift, kl_shift, s); break;
+    case 16: launch_typed<T, CPL, 16>(a, grid, n_light, g_shift, kl_shift, s); break;
+    default: launch_typed<T, CPL, 8>(a, grid, n_light, g_shift, kl_shift, s); break;
+  }
 }
 
 int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
   if (int rc = validate(a)) return rc;
   const int G = a.kl * a.es;
   const int g_shift = ilog2(G), kl_shift = ilog2(a.kl);
-  const int64_t rows_per_block = (int64_t)kWavesPerBlock * (64 / G);
-  const int64_t n_light = (a.row_end - a.row_begin + rows_per_block - 1) / rows_per_block;
-  const int64_t gx = n_light + a.num_heavy;
+  int64_t n_light = 0, gx = 0;
+  if (a.flags & GRG_HIP_FLAG_PARTIALS) {
+    gx = a.split_items;
+  } else {
+    const int64_t rows_per_block = (int64_t)kWavesPerBlock * (64 / G);
+    n_light = (a.row_end - a.row_begin + rows_per_block - 1) / rows_per_block;
+    gx = n_light + a.num_heavy;
+  }
   if (gx == 0) return 0;
   if (gx > 0x7fffffffLL) return fail("grid too large");
   const int chunk = a.kl * a.cpl;
   dim3 grid((unsigned)gx, (unsigned)((a.k + chunk - 1) / chunk));
   if (a.dtype == GRG_HIP_F64) {
     switch (a.cpl) {
-      case 1: launch_typed<double, 1, false>(a, grid, n_light, g_shift, kl_shift, stream); break;
-      case 2: launch_typed<double, 2, false>(a, grid, n_light, g_shift, kl_shift, stream); break;
-      default: launch_typed<double, 4, false>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      case 1: launch_unrolled<double, 1>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      case 2: launch_unrolled<double, 2>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      default: launch_unrolled<double, 4>(a, grid, n_light, g_shift, kl_shift, stream); break;
     }
   } else {
     switch (a.cpl) {
-      case 1: launch_typed<float, 1, true>(a, grid, n_light, g_shift, kl_shift, stream); break;
-      case 2: launch_typed<float, 2, true>(a, grid, n_light, g_shift, kl_shift, stream); break;
-      default: launch_typed<float, 4, true>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      case 1: launch_unrolled<float, 1>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      case 2: launch_unrolled<float, 2>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      default: launch_unrolled<float, 4>(a, grid, n_light, g_shift, kl_shift, stream); break;
     }
   }
   hipError_t err = hipGetLastError();
@@ -391,6 +479,15 @@ int grg_hip_stream_device(void* stream, int32_t* device) {
   hipError_t err = hipStreamGetDevice(static_cast<hipStream_t>(stream), &d);
   if (err != hipSuccess) return fail(std::string("hipStreamGetDevice: ") + hipGetErrorString(err), 2);
   *device = d;
+  return 0;
+}
+
+int grg_hip_stream_capturing(void* stream, int32_t* capturing) {
+  if (!capturing) return fail("capturing is null");
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  hipError_t err = hipStreamIsCapturing(static_cast<hipStream_t>(stream), &st);
+  if (err != hipSuccess) return fail(std::string("hipStreamIsCapturing: ") + hipGetErrorString(err), 2);
+  *capturing = st == hipStreamCaptureStatusNone ? 0 : 1;
   return 0;
 }
 

@@ -13,10 +13,11 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 F64 = 0
 F32C = 1
 FLAG_XCD_REMAP = 2
+FLAG_PARTIALS = 4
 MAX_SEGS = 3
 MAX_MAPS = 2
 
@@ -39,6 +40,7 @@ class GrgHipGatherArgs(ctypes.Structure):
         ("cpl", c_int32),
         ("kl", c_int32),
         ("es", c_int32),
+        ("unroll", c_int32),
         ("flags", c_int32),
         ("out", c_void_p),
         ("ld_out", c_int64),
@@ -53,6 +55,11 @@ class GrgHipGatherArgs(ctypes.Structure):
         ("heavy_threshold", c_int64),
         ("heavy_rows", c_void_p),
         ("num_heavy", c_int64),
+        ("split_ptr", c_void_p),
+        ("split_row", c_void_p),
+        ("split_items", c_int64),
+        ("split_chunk", c_int64),
+        ("partials", c_void_p),
     ]
 
 
@@ -75,6 +82,8 @@ class HipLibrary:
         lib.grg_hip_device_count.restype = ctypes.c_int
         lib.grg_hip_stream_device.argtypes = [c_void_p, ctypes.POINTER(c_int32)]
         lib.grg_hip_stream_device.restype = ctypes.c_int
+        lib.grg_hip_stream_capturing.argtypes = [c_void_p, ctypes.POINTER(c_int32)]
+        lib.grg_hip_stream_capturing.restype = ctypes.c_int
         version = int(lib.grg_hip_abi_version())
         if version != ABI_VERSION:
             raise RuntimeError(f"libgrg_hip.so ABI version {version} != expected {ABI_VERSION}; rebuild the extension")
@@ -101,6 +110,11 @@ class HipLibrary:
         out = c_int32(-1)
         self._check(self.lib.grg_hip_stream_device(c_void_p(int(stream_ptr)), ctypes.byref(out)), "grg_hip_stream_device")
         return int(out.value)
+
+    def stream_capturing(self, stream_ptr: int) -> bool:
+        out = c_int32(0)
+        self._check(self.lib.grg_hip_stream_capturing(c_void_p(int(stream_ptr)), ctypes.byref(out)), "grg_hip_stream_capturing")
+        return bool(out.value)
 
 
 _LIB: HipLibrary | None = None

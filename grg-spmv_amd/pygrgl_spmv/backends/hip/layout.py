@@ -39,6 +39,10 @@ from pygrgl_spmv.grg.artifact import ArtifactScan, scan_grg_spmv
 # Rows of gather degree above this (or above 8x their level's typical batch) get a
 # workgroup of their own.
 HEAVY_THRESHOLD = 512
+# Rows of gather degree above SPLIT_MIN are cut into chunks of SPLIT_CHUNK entries,
+# one workgroup each, whose FP64 partial rows the level launch then sums.
+SPLIT_MIN = 4096
+SPLIT_CHUNK = 2048
 
 
 @dataclass
@@ -222,15 +226,28 @@ def _metadata_bytes(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequireme
     return int(total)
 
 
-def schedule_capacity_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequirements) -> int:
-    """int32 entries reserved for row-batch tables and heavy-row lists.
+def _gather_entries_bound(scan: ArtifactScan) -> int:
+    """Upper bound of the gather entries of any one sweep configuration."""
+    return int(scan.num_edges + scan.selector_mut_nnz + scan.selector_miss_nnz)
 
-    Every sweep configuration (up; down; down with missing-data input) lists
-    each node either inside one batch or as a heavy row, and a batch holds at
-    least one row, so one configuration needs at most ``num_nodes`` entries.
+
+def split_items_bound(scan: ArtifactScan) -> int:
+    """Upper bound of the split items of one level launch: a split row of degree
+    d > SPLIT_MIN has ceil(d / SPLIT_CHUNK) < d / SPLIT_CHUNK + d / SPLIT_MIN items."""
+    e = _gather_entries_bound(scan)
+    return int(e // SPLIT_CHUNK + e // SPLIT_MIN + 1)
+
+
+def schedule_capacity_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequirements) -> int:
+    """int32 entries reserved for heavy-row lists and split-row tables.
+
+    Per sweep configuration (up; down; down with missing-data input): at most
+    ``num_nodes`` heavy rows, one item offset per heavy row plus one per level,
+    and the split items of all levels.
     """
     configs = (pair.plan_up is not None) + (pair.plan_down is not None) * (2 if req.need_down_miss_input else 1)
-    return int(configs * scan.num_nodes)
+    per_config = 2 * scan.num_nodes + scan.num_levels + split_items_bound(scan)
+    return int(configs * per_config)
 
 
 def plan_hip_layout(
@@ -276,6 +293,7 @@ def plan_hip_layout(
 
     selector_bytes = metadata_bytes = xtx_bytes = heavy_bytes = 0
     heavy_caps = []
+    items_max = 0
     for s in scans:
         selector_bytes += _selector_bytes(s, pair, requirements)
         metadata_bytes += _metadata_bytes(s, pair, requirements, dtype)
@@ -284,7 +302,9 @@ def plan_hip_layout(
         cap = schedule_capacity_for(s, pair, requirements)
         heavy_caps.append(cap)
         heavy_bytes += 4 * cap
+        items_max = max(items_max, split_items_bound(s))
     max_k = enabled_max_k(pair, requirements)
+    scratch_bytes = 8 * items_max * max_k  # FP64 partial rows of split rows, reused level by level
     state_bytes = N_max * max_k * dtype.itemsize
     io0_rows = max(S_max, M_max)
     io1_rows = side_io_rows(pair, requirements, max_num_mutations=M_max, max_num_nodes=N_max)
@@ -305,7 +325,7 @@ def plan_hip_layout(
             blocks_up=ups, blocks_down=downs,
         ))
 
-    fixed = int(selector_bytes + metadata_bytes + xtx_bytes + heavy_bytes + dense_bytes + state_bytes)
+    fixed = int(selector_bytes + metadata_bytes + xtx_bytes + heavy_bytes + dense_bytes + state_bytes + scratch_bytes)
     full_resident = int(sum(b.nbytes for b in blocks))
     required_full = fixed + full_resident
     effective = required_full if budget == 0 else budget
@@ -359,7 +379,7 @@ def plan_hip_layout(
         "schedule": int(heavy_bytes),
         "dense_arena": int(dense_bytes),
         "state": int(state_bytes),
-        "scratch": 0,
+        "scratch": int(scratch_bytes),
     }
     items = [BudgetItem(kind="fixed", name=name, nbytes=int(by_cat[name]))
              for name in ("selectors", "metadata", "xtx_bias", "schedule", "dense_arena", "state", "scratch")]
@@ -383,4 +403,4 @@ def plan_hip_layout(
     )
 
 
-__all__ = ["HEAVY_THRESHOLD", "HipArtifactLayout", "HipBlockPlan", "HipLayout", "HipSlotPlan", "enabled_max_k", "plan_hip_layout", "side_io_rows"]
+__all__ = ["HEAVY_THRESHOLD", "SPLIT_CHUNK", "SPLIT_MIN", "split_items_bound", "HipArtifactLayout", "HipBlockPlan", "HipLayout", "HipSlotPlan", "enabled_max_k", "plan_hip_layout", "side_io_rows"]

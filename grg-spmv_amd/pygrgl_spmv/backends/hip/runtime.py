@@ -8,7 +8,8 @@ with the same protocol towards ``BoundGRG``: everything is allocated in
 ``miss_input`` / ``miss_output`` / ``init_vector`` / ``init_matrix`` tensors
 are transposed views of one shared arena (so one direction's output is the
 other direction's input); ``op()`` performs no allocation, host sync or
-device read-back and can be captured into a hipGraph.
+device read-back and, when every block is resident, can be captured into a
+hipGraph.
 
 Execution differs from the reference by design:
 
@@ -41,7 +42,7 @@ import scipy.sparse as sp
 
 from pygrgl_spmv.backends.base import copy_struct_checked, relink_stream_dependencies
 from pygrgl_spmv.backends.hip import ffi
-from pygrgl_spmv.backends.hip.layout import HipLayout, enabled_max_k, side_io_rows
+from pygrgl_spmv.backends.hip.layout import SPLIT_CHUNK, SPLIT_MIN, HipLayout, enabled_max_k, side_io_rows
 from pygrgl_spmv.backends.types import Direction, InitMode
 from pygrgl_spmv.grg import BoundGRG, _CudaMatmulSpec
 from pygrgl_spmv.grg.artifact import _load_grg_spmv_host, iter_artifact_blocks
@@ -95,27 +96,55 @@ def launch_shape(dtype_code: int, k: int, lds, mean_degree: float) -> tuple[int,
 class _LevelSchedule:
     """Heavy rows of one level launch: rows of gather degree > ``heavy_threshold``
     are produced by a workgroup each and listed at ``heavy_offset`` in the
-    schedule buffer; ``mean_degree`` is the mean over the other rows."""
+    schedule buffer; ``mean_degree`` is the mean over the other rows. Heavy
+    rows of degree > split_min are split: ``num_items`` chunks of
+    ``split_chunk`` entries, with item offsets per heavy row at
+    ``split_ptr_offset`` and the heavy index of each item at
+    ``split_row_offset`` (all int32 entries of the schedule buffer)."""
 
     heavy_offset: int
     num_heavy: int
     heavy_threshold: int
     mean_degree: float
+    split_ptr_offset: int = 0
+    split_row_offset: int = 0
+    num_items: int = 0
+    split_chunk: int = 0
 
 
-def level_schedule(deg: np.ndarray, lo: int, hi: int, *, heavy_min: int, cursor: int):
+def level_schedule(deg: np.ndarray, lo: int, hi: int, *, heavy_min: int, cursor: int,
+                   split_min: int = 0, split_chunk: int = 0):
     """Schedule for rows [lo, hi) of one sweep configuration (``deg`` = gather degree per row).
 
     Rows above max(heavy_min, 8 x the level's mean degree) get a 256-thread
-    workgroup of their own, so one long row does not hold a wave of short ones.
+    workgroup of their own, so one long row does not hold a wave of short ones;
+    rows above ``split_min`` (when > 0) are further cut into chunks of
+    ``split_chunk`` entries. Returns the schedule and its int32 table
+    (heavy rows, then split item offsets and item rows when any row is split),
+    which the caller stores at ``cursor``.
     """
     seg = deg[lo:hi]
     mean = float(seg.mean()) if seg.size else 0.0
     heavy = int(max(heavy_min, 8.0 * mean))
-    heavy_rows = (np.flatnonzero(seg > heavy) + lo).astype(np.int32)
+    if split_min > 0:
+        heavy = min(heavy, int(split_min))  # rows to split are always heavy rows
+    heavy_idx = np.flatnonzero(seg > heavy)
+    heavy_rows = (heavy_idx + lo).astype(np.int32)
     light = seg[seg <= heavy]
     light_mean = max(1.0, float(light.mean())) if light.size else 1.0
-    return _LevelSchedule(cursor, int(heavy_rows.size), heavy, light_mean), heavy_rows
+    n_heavy = int(heavy_rows.size)
+    if split_min > 0 and n_heavy:
+        hdeg = seg[heavy_idx].astype(np.int64)
+        items = np.where(hdeg > split_min, -(-hdeg // split_chunk), 0)
+        if items.any():
+            split_ptr = np.zeros(n_heavy + 1, dtype=np.int64)
+            np.cumsum(items, out=split_ptr[1:])
+            n_items = int(split_ptr[-1])
+            split_row = np.repeat(np.arange(n_heavy, dtype=np.int64), items)
+            table = np.concatenate([heavy_rows, split_ptr.astype(np.int32), split_row.astype(np.int32)])
+            return _LevelSchedule(cursor, n_heavy, heavy, light_mean, cursor + n_heavy, cursor + 2 * n_heavy + 1,
+                                  n_items, int(split_chunk)), table
+    return _LevelSchedule(cursor, n_heavy, heavy, light_mean), heavy_rows
 
 
 # --------------------------------------------------------------------------
@@ -148,6 +177,7 @@ class _HipArtifact:
     down_stream: list = field(default_factory=list)
     up_events: list = field(default_factory=list)     # per level: list[(copy_done, compute_done)]
     down_events: list = field(default_factory=list)
+    max_items: int = 0                                # largest split-item count of one level launch
 
 
 def _i32(values, label: str) -> np.ndarray:
@@ -220,6 +250,13 @@ class _HipPreparedMatmul:
     def __exit__(self, exc_type, exc, tb) -> None:
         return None
 
+    @property
+    def capturable(self) -> bool:
+        """True when ``op()`` can be captured into a hipGraph: every block of the
+        sweep is resident. Sweeps that stream blocks through the pinned ring run
+        eagerly only (capturing them is refused with a RuntimeError)."""
+        return not any(it[0] == "stream" for it in self._steps)
+
     def __call__(self) -> None:
         with self._runtime._call_scope():
             self._call_prelocked()
@@ -231,7 +268,15 @@ class _HipPreparedMatmul:
 class HipRuntime:
     """Runtime-owned HIP execution with resident or streamed sparse blocks."""
 
-    def __init__(self, layout: HipLayout, *, autotune: bool = True) -> None:
+    def __init__(self, layout: HipLayout, *, autotune: bool = True, split_min: int = SPLIT_MIN,
+                 split_chunk: int = SPLIT_CHUNK) -> None:
+        """``autotune``: time the admissible launch shapes of every level launch when an
+        op is prepared (otherwise a fixed default shape). ``split_min`` /
+        ``split_chunk``: rows of gather degree above ``split_min`` are summed in
+        chunks of ``split_chunk`` entries by separate workgroups (the layout's
+        scratch bound assumes the defaults)."""
+        if int(split_min) < 1 or int(split_chunk) < 1:
+            raise ValueError("split_min and split_chunk must be >= 1")
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError("HIP runtime requires a ROCm GPU (torch.cuda.is_available() is False)")
@@ -240,6 +285,8 @@ class HipRuntime:
         self.device = torch.device("cuda", int(layout.device))
         self.stream_ptr = int(layout.stream_ptr)
         self.autotune = bool(autotune)
+        self.split_min = int(split_min)
+        self.split_chunk = int(split_chunk)
         self._dtype_code = ffi.F64 if np.dtype(layout.dtype) == np.float64 else ffi.F32C
         self._reset_owned_state()
 
@@ -259,6 +306,7 @@ class HipRuntime:
         self._io0 = None
         self._io1 = None
         self._aux = None
+        self._partials = None
         self._artifacts = ()
         self._grgs = ()
         self._tune_cache = {}
@@ -310,6 +358,10 @@ class HipRuntime:
             for ai, (al, st) in enumerate(zip(lay.artifacts, states)):
                 arts.append(self._build_artifact(ai, al, st))
             self._artifacts = tuple(arts)
+            items = max([a.max_items for a in arts] + [1])
+            with torch.cuda.device(self.device):
+                # FP64 partial rows of split rows, reused by every level launch in stream order
+                self._partials = torch.zeros((items * max_k,), dtype=torch.float64, device=self.device)
             torch.cuda.synchronize(self.device)
             self._grgs = tuple(BoundGRG(self, i, a.state, a.path, lay.device) for i, a in enumerate(self._artifacts))
             self._entered = True
@@ -554,8 +606,10 @@ class HipRuntime:
             cursor = 0
             for name, deg in configs.items():
                 for l in range(L):
-                    sched, h = level_schedule(deg, int(off[l]), int(off[l + 1]), heavy_min=int(lay.heavy_threshold), cursor=cursor)
+                    sched, h = level_schedule(deg, int(off[l]), int(off[l + 1]), heavy_min=int(lay.heavy_threshold), cursor=cursor,
+                                              split_min=self.split_min, split_chunk=self.split_chunk)
                     art.sched[(name, l)] = sched
+                    art.max_items = max(art.max_items, sched.num_items)
                     tables.append(h)
                     cursor += h.size
             cap = lay.schedule_capacity[ai] if ai < len(lay.schedule_capacity) else cursor
@@ -587,9 +641,15 @@ class HipRuntime:
             mean_degree = sched.mean_degree
             a.heavy_threshold = sched.heavy_threshold
             a.heavy_rows, a.num_heavy = sched_base + 4 * sched.heavy_offset, sched.num_heavy
+            if sched.num_items:
+                a.split_ptr = sched_base + 4 * sched.split_ptr_offset
+                a.split_row = sched_base + 4 * sched.split_row_offset
+                a.split_items, a.split_chunk = sched.num_items, sched.split_chunk
+                a.partials = self._partials.data_ptr()
         else:
             a.heavy_threshold, a.heavy_rows, a.num_heavy = -1, 0, 0
         a.flags = 0
+        a.unroll = KERNEL_UNROLL
         lds = [ld_out] + [s[3] for s in segs] + [m[2] for m in maps]
         a.cpl, a.kl, a.es = launch_shape(self._dtype_code, int(k), lds, mean_degree)
         a.out, a.ld_out, a.out_base = int(out), int(ld_out), int(out_base)
@@ -604,40 +664,54 @@ class HipRuntime:
         a.col_vec = int(col_vec or 0)
         return a
 
+    def _time_launch(self, args: ffi.GrgHipGatherArgs, reps: int = 3) -> float:
+        """Best-of-``reps`` device time (ms) of one launch on the root stream, after a warm launch."""
+        torch = _torch()
+        root = self._root_stream
+        ptr = int(root.cuda_stream)
+        self._lib.gather(args, ptr)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        ev[0].record(root)
+        for i in range(reps):
+            self._lib.gather(args, ptr)
+            ev[i + 1].record(root)
+        ev[-1].synchronize()
+        return min(ev[i].elapsed_time(ev[i + 1]) for i in range(reps))
+
     def _tune(self, a: ffi.GrgHipGatherArgs, key) -> None:
-        """Per-level autotuner: time every admissible (cpl, kl, es, XCD remap) launch
-        shape of this level launch on the device and keep the fastest (cached per
-        artifact, sweep configuration, level and k). Run at prepare time, never in
-        ``op()``: the timing launches write only the level's node rows."""
+        """Per-level autotuner, run when an op is prepared (never in ``op()``; the
+        timing launches write only the level's node rows). Stage 1 times every
+        admissible (cpl, kl, es) shape at 8 gathers in flight per lane; stage 2
+        tries 4 and 16 gathers in flight and the XCD remap on the best shape. The
+        fastest launch is cached per artifact, sweep configuration, level and k."""
         torch = _torch()
         best = self._tune_cache.get(key)
         if best is None and self.autotune and a.row_end > a.row_begin:
             lds = [a.ld_out] + [a.seg[i].ld for i in range(a.nseg)] + [a.map[i].ld for i in range(a.nmap)]
-            cands = [(cpl, kl, es, fl)
-                     for cpl, kl in column_shapes(a.dtype, a.k, lds)
-                     for es in (1, 2, 4, 8, 16, 32, 64) if kl * es <= 64
-                     for fl in (0, ffi.FLAG_XCD_REMAP)]
-            root = self._root_stream
-            ptr = int(root.cuda_stream)
             trial = ffi.GrgHipGatherArgs()
             ctypes.pointer(trial)[0] = a
+            elem = 8 if a.dtype == ffi.F64 else 4
             timings = []
             with torch.cuda.device(self.device):
-                for cpl, kl, es, fl in cands:
-                    trial.cpl, trial.kl, trial.es, trial.flags = cpl, kl, es, fl
-                    self._lib.gather(trial, ptr)  # warm
-                    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-                    ev[0].record(root)
-                    for i in range(3):
-                        self._lib.gather(trial, ptr)
-                        ev[i + 1].record(root)
-                    ev[-1].synchronize()
-                    t = min(ev[i].elapsed_time(ev[i + 1]) for i in range(3))
-                    timings.append((t, cpl, kl, es, fl))
+                for cpl, kl in column_shapes(a.dtype, a.k, lds):
+                    for es in (1, 2, 4, 8, 16, 32, 64):
+                        if kl * es > 64:
+                            continue
+                        trial.cpl, trial.kl, trial.es, trial.unroll, trial.flags = cpl, kl, es, 8, 0
+                        timings.append((self._time_launch(trial), cpl, kl, es, 8, 0))
+                _, cpl, kl, es, _, _ = min(timings)
+                for unroll in (4, 8, 16):
+                    if unroll * cpl * elem > 256:  # keep the loads in flight within 64 registers
+                        continue
+                    for fl in (0, ffi.FLAG_XCD_REMAP):
+                        if (unroll, fl) == (8, 0):
+                            continue
+                        trial.cpl, trial.kl, trial.es, trial.unroll, trial.flags = cpl, kl, es, unroll, fl
+                        timings.append((self._time_launch(trial), cpl, kl, es, unroll, fl))
             best = min(timings)[1:]
             self._tune_cache[key] = best
         if best is not None:
-            a.cpl, a.kl, a.es, a.flags = best
+            a.cpl, a.kl, a.es, a.unroll, a.flags = best
 
     def _build_steps(self, art: _HipArtifact, spec: _CudaMatmulSpec):
         """The prepared sweep: a list of ("batch", array, n) and ("stream", step, args, events) items."""
@@ -702,6 +776,12 @@ class HipRuntime:
                                     row_scale=row_scale_init, col_vec=col_vec_init, sched=sched, sched_base=sched_base)
             if segs:
                 self._tune(level_args, (art.path, cfg, l, k))
+            if sched is not None and sched.num_items:
+                # partial rows of the level's split rows, same column shape as the level launch
+                part = ffi.GrgHipGatherArgs()
+                ctypes.pointer(part)[0] = level_args
+                part.flags = ffi.FLAG_PARTIALS
+                batch.append(part)
             batch.append(level_args)
             for i, step in enumerate(streams[l]):
                 flush()
@@ -760,6 +840,9 @@ class HipRuntime:
             root = self._root_stream
             root_ptr = int(root.cuda_stream)
             has_stream = any(it[0] == "stream" for it in items)
+            if has_stream and self._lib.stream_capturing(int(self._caller_stream.cuda_stream)):
+                raise RuntimeError("HIP runtime: a sweep that streams blocks through the pinned ring cannot be captured "
+                                   "into a hipGraph; plan a layout with every block resident to capture it")
             timer = self._sweep_timer
             if timer is not None:
                 t_start, t_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

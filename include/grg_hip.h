@@ -37,11 +37,11 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 3
+#define GRG_HIP_ABI_VERSION 4
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
-  GRG_HIP_F32C = 1      /* FP32 values, compensated (two-float) accumulation per row */
+  GRG_HIP_F32C = 1      /* FP32 values, two-float compensated (TwoSum) accumulation per row */
 };
 
 #define GRG_HIP_MAX_SEGS 3
@@ -50,6 +50,11 @@ enum GrgHipDtype {
 /* Workgroups are dispatched round-robin over the 8 XCDs; remap them so each
  * XCD (and its L2) works on one contiguous eighth of the rows. */
 #define GRG_HIP_FLAG_XCD_REMAP 2
+/* Partials launch of split heavy rows: grid = split_items workgroups; item i
+ * sums entries [j * split_chunk, (j + 1) * split_chunk) of the concatenated
+ * segments of heavy row h = split_row[i] (j = i - split_ptr[h]) and writes
+ * the FP64 row partials[i * k + c]. No row terms, nothing else written. */
+#define GRG_HIP_FLAG_PARTIALS 4
 
 /* sum_{j in [ptr[row - ptr_base], ptr[row - ptr_base + 1])} src[idx[j] * ld + c] */
 typedef struct GrgHipSeg {
@@ -78,6 +83,7 @@ typedef struct GrgHipGatherArgs {
   int32_t cpl;          /* columns per lane: 1, 2 or 4; k, every ld and every base must be multiples */
   int32_t kl;           /* lanes across one column chunk (power of two, kl * cpl columns per chunk) */
   int32_t es;           /* lanes interleaving over one row's entries (power of two, kl * es <= 64) */
+  int32_t unroll;       /* gathers in flight per lane and round: 4, 8 or 16 (0 means 8) */
   int32_t flags;        /* GRG_HIP_FLAG_* launch options */
   void* out;            /* out row r lives at out + (r - out_base) * ld_out */
   int64_t ld_out;
@@ -96,6 +102,16 @@ typedef struct GrgHipGatherArgs {
   int64_t heavy_threshold;
   const int32_t* heavy_rows;
   int64_t num_heavy;
+  /* Split heavy rows (optional). When split_ptr is set, heavy row h with a
+   * non-empty item range [split_ptr[h], split_ptr[h + 1]) takes the sum of the
+   * partial rows of that range (written by a preceding GRG_HIP_FLAG_PARTIALS
+   * launch with the same arguments) in place of its segment sums; its row
+   * terms are added as usual. */
+  const int32_t* split_ptr;   /* num_heavy + 1 item offsets */
+  const int32_t* split_row;   /* split_items heavy indices (partials launch) */
+  int64_t split_items;
+  int64_t split_chunk;        /* segment entries per item */
+  void* partials;             /* double, split_items x k */
 } GrgHipGatherArgs;
 
 /* One row-gather launch (see above). Replaces one level of the reference's
@@ -104,7 +120,8 @@ typedef struct GrgHipGatherArgs {
 int grg_hip_gather(const GrgHipGatherArgs* args, void* stream);
 
 /* n row-gather launches in order on one stream (a whole prepared sweep:
- * seed-fused levels then write-out), from one host call. */
+ * seed-fused levels then write-out, with the partials launch of a level's
+ * split rows right before the level), from one host call. */
 int grg_hip_gather_batch(const GrgHipGatherArgs* args, int32_t n, void* stream);
 
 /* Thread-local description of the last failure ("" if none). */
@@ -119,6 +136,11 @@ int grg_hip_device_count(int32_t* count);
 
 /* hipStreamGetDevice (replaces backends/_cuda_stream.py:127-137). */
 int grg_hip_stream_device(void* stream, int32_t* device);
+
+/* *capturing = 1 while the stream is being captured into a hipGraph, else 0
+ * (hipStreamIsCapturing). No reference counterpart: the runtime uses it to
+ * refuse capturing sweeps that stream blocks through the pinned ring. */
+int grg_hip_stream_capturing(void* stream, int32_t* capturing);
 
 #ifdef __cplusplus
 }

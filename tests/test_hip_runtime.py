@@ -1,0 +1,368 @@
+"""HIP runtime behaviour on the GPU: split rows, untuned launches, streamed ring
+layouts, hipGraph capture, several GRGs in one runtime, lifecycle.
+
+Counterparts in the reference: streaming and budget cases of
+``runtime/test_triton.py:230-576`` and ``_streaming_cases.py``, graph capture
+``test_triton.py:920-1058``, runtime protocol ``runtime/test_prepare_cuda.py``.
+Every numeric check compares against the CPU oracle on the same inputs: FP64
+within 1e-12 normwise relative error, exactly for +-1 / integer inputs.
+"""
+
+from __future__ import annotations
+
+from contextlib import ExitStack
+
+import numpy as np
+import pytest
+
+from oracle import grg_oracle as O
+from tests._builders import build_hip_layout, full_requirements, write_band_artifact, write_synthetic_artifact
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _cfg(seed, samples=300, mutations=2000, ploidy=2, missing=0.05):
+    from pygrgl_spmv.synth import SynthConfig
+
+    return SynthConfig(num_samples=samples, num_mutations=mutations, seed=seed, ploidy=ploidy, num_trees=120,
+                       keep_fraction=0.3, window_fraction=0.1, num_segments=3, missing_fraction=missing, missing_pool=6)
+
+
+@pytest.fixture(scope="module")
+def art_a(artifact_dir):
+    return write_synthetic_artifact(artifact_dir, "runtime-a", _cfg(201))
+
+
+@pytest.fixture(scope="module")
+def art_b(artifact_dir):
+    return write_synthetic_artifact(artifact_dir, "runtime-b", _cfg(202, samples=180, mutations=900, ploidy=1, missing=0.0))
+
+
+@pytest.fixture(scope="module")
+def oracle_a(art_a):
+    return O.load_artifact(art_a)
+
+
+def _cols(g, direction, by_individual=False):
+    if direction == "up":
+        return g.num_individuals if by_individual else g.num_samples
+    return g.num_mutations
+
+
+def _check(got, exp):
+    assert got.shape == exp.shape
+    assert O.normwise_rel_err(got, exp) <= FP64_TOL
+
+
+# ---------------------------------------------------------------------------
+# split rows (partials launch + level launch)
+# ---------------------------------------------------------------------------
+
+
+@pytest.fixture(scope="module")
+def split_runtime(art_a):
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=8, max_k_down=8))
+    with HipRuntime(layout, split_min=12, split_chunk=5) as rt:
+        yield rt
+
+
+def test_split_schedule_present(split_runtime):
+    art = split_runtime._artifacts[0]
+    items = [s.num_items for s in art.sched.values()]
+    assert max(items) > 0, "the small split parameters must split some rows"
+
+
+@pytest.mark.parametrize("direction", ["up", "down"])
+@pytest.mark.parametrize("init_mode", ["none", "xtx", "matrix"])
+def test_split_rows_match_oracle(split_runtime, oracle_a, direction, init_mode):
+    (grg,) = split_runtime.grgs
+    rng = np.random.default_rng(3100 + len(init_mode) + (7 if direction == "down" else 0))
+    x = rng.standard_normal((3, _cols(oracle_a, direction)))
+    init = {"none": None, "xtx": "xtx", "matrix": rng.standard_normal((3, oracle_a.num_nodes))}[init_mode]
+    _check(grg.matmul(x, direction, init=init), O.matmul(oracle_a, x, direction, init=init))
+
+
+def test_split_rows_missing_down_and_exact(split_runtime, oracle_a):
+    (grg,) = split_runtime.grgs
+    rng = np.random.default_rng(3200)
+    y = rng.choice(np.array([-1.0, 1.0]), size=(5, oracle_a.num_mutations))
+    miss = rng.choice(np.array([-1.0, 1.0]), size=(5, oracle_a.num_mutations))
+    np.testing.assert_array_equal(grg.matmul(y, "down", miss=miss.copy()), O.matmul(oracle_a, y, "down", miss=miss.copy()))
+    x = rng.choice(np.array([-1.0, 1.0]), size=(8, oracle_a.num_individuals))
+    np.testing.assert_array_equal(grg.matmul(x, "up", by_individual=True), O.matmul(oracle_a, x, "up", by_individual=True))
+
+
+# ---------------------------------------------------------------------------
+# default launch shapes (no autotuning)
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("k", [1, 4, 9])
+def test_untuned_launch_shapes(art_a, oracle_a, k):
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=16, max_k_down=16))
+    with HipRuntime(layout, autotune=False) as rt:
+        (grg,) = rt.grgs
+        rng = np.random.default_rng(3300 + k)
+        for direction in ("up", "down"):
+            x = rng.standard_normal((k, _cols(oracle_a, direction)))
+            _check(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
+
+
+# ---------------------------------------------------------------------------
+# streamed blocks through the pinned ring
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("ring", [1, 2, 4])
+def test_all_streamed_ring(art_a, oracle_a, ring):
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=4, max_k_down=4),
+                              ring_buffer_size=ring, allow_residency=False)
+    assert not any(b.resident for a in layout.artifacts for b in (*a.blocks_up, *a.blocks_down))
+    with HipRuntime(layout) as rt:
+        (grg,) = rt.grgs
+        rng = np.random.default_rng(3400 + ring)
+        for direction in ("up", "down"):
+            x = rng.standard_normal((4, _cols(oracle_a, direction)))
+            _check(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
+        y = rng.standard_normal((2, oracle_a.num_mutations))
+        miss = rng.standard_normal((2, oracle_a.num_mutations))
+        _check(grg.matmul(y, "down", miss=miss), O.matmul(oracle_a, y, "down", miss=miss))
+        x = rng.standard_normal((2, oracle_a.num_samples))
+        _check(grg.matmul(x, "up", init="xtx"), O.matmul(oracle_a, x, "up", init="xtx"))
+
+
+def test_partially_resident_budget(art_a, oracle_a):
+    """A budget between the smallest streamed layout and full residency keeps some
+    blocks resident and streams the rest (planner rules as triton/backend.py:439-499)."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    req = full_requirements(max_k_up=2, max_k_down=2)
+    full = build_hip_layout([art_a], requirements=req, vram_budget_bytes=0)
+    streamed = build_hip_layout([art_a], requirements=req, ring_buffer_size=2, allow_residency=False)
+    budget = (streamed.bytes_total + full.required_budget_for_full_residency) // 2
+    layout = build_hip_layout([art_a], requirements=req, vram_budget_bytes=budget, ring_buffer_size=2)
+    flags = [b.resident for a in layout.artifacts for b in (*a.blocks_up, *a.blocks_down)]
+    assert any(flags) and not all(flags)
+    with HipRuntime(layout) as rt:
+        (grg,) = rt.grgs
+        rng = np.random.default_rng(3500)
+        for direction in ("up", "down"):
+            x = rng.choice(np.array([-1.0, 1.0]), size=(2, _cols(oracle_a, direction)))
+            np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
+
+
+def test_streamed_band_closed_form(artifact_dir):
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    n, bw = 64, 8
+    art = write_band_artifact(artifact_dir, "band-n64", n=n, bandwidth=bw)
+    layout = build_hip_layout([art], requirements=full_requirements(max_k_up=1, max_k_down=1),
+                              ring_buffer_size=2, allow_residency=False)
+    with HipRuntime(layout) as rt:
+        (grg,) = rt.grgs
+        x = np.random.default_rng(3600).choice(np.array([-1.0, 1.0]), size=(1, n))
+        np.testing.assert_array_equal(grg.matmul(x, "up"), O.band_expected_up(x.T, shifts=(0, bw, 2 * bw), bandwidth=bw).T)
+        np.testing.assert_array_equal(grg.matmul(x, "down"), O.band_expected_down(x.T, shifts=(0, bw, 2 * bw), bandwidth=bw).T)
+
+
+# ---------------------------------------------------------------------------
+# hipGraph capture of prepared op chains
+# ---------------------------------------------------------------------------
+
+
+def _chain_expected(g, order, src, init_first, init_second):
+    mid = O.matmul(g, src, order[0], init=init_first)
+    return O.matmul(g, mid, order[1], init=init_second)
+
+
+def _enter_chain(stack, grg, order, k):
+    first = stack.enter_context(grg.prepare_matmul_cuda(direction=order[0], k=k, init_mode="vector"))
+    second = stack.enter_context(grg.prepare_matmul_cuda(direction=order[1], k=k, init_mode="vector"))
+    assert first.output.data_ptr() == second.input.data_ptr()
+    return first, second
+
+
+def _run_chain(first, second, src, i1, i2, result):
+    first.input.copy_(src)
+    first.init_vector.copy_(i1)
+    first()
+    second.init_vector.copy_(i2)
+    second()
+    result.copy_(second.output)
+
+
+def test_streamed_ops_are_not_capturable(art_a):
+    """Sweeps with streamed blocks run eagerly only; resident sweeps are capturable."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    req = full_requirements(max_k_up=1, max_k_down=1)
+    for streamed in (False, True):
+        layout = build_hip_layout([art_a], requirements=req, ring_buffer_size=2 if streamed else 0, allow_residency=not streamed)
+        with HipRuntime(layout) as rt:
+            (grg,) = rt.grgs
+            for direction in ("up", "down"):
+                assert grg.prepare_matmul_cuda(direction=direction, k=1).capturable is (not streamed)
+
+
+def test_graph_capture_chain_replays(art_a, oracle_a):
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    with torch.cuda.device(0):
+        cap = torch.cuda.Stream()
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=2, max_k_down=2), stream=cap)
+    order = ("down", "up")
+    rng = np.random.default_rng(3700)
+    src_np = rng.standard_normal((2, _cols(oracle_a, order[0])))
+    i1_np, i2_np = rng.standard_normal(2), rng.standard_normal(2)
+    expected = _chain_expected(oracle_a, order, src_np, i1_np, i2_np)
+    with HipRuntime(layout) as rt, ExitStack() as stack:
+        (grg,) = rt.grgs
+        first, second = _enter_chain(stack, grg, order, 2)
+        dev = first.input.device
+        src = torch.from_numpy(src_np).to(dev)
+        i1, i2 = torch.from_numpy(i1_np).to(dev), torch.from_numpy(i2_np).to(dev)
+        result = torch.empty_like(second.output)
+        # eager reference run on the capture stream
+        with torch.cuda.stream(cap):
+            _run_chain(first, second, src, i1, i2, result)
+        torch.cuda.synchronize()
+        eager = result.cpu().numpy().copy()
+        _check(eager, expected)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=cap):
+            _run_chain(first, second, src, i1, i2, result)
+        for rep in range(3):
+            # unrelated eager work on the arena between replays
+            first.input.zero_()
+            torch.cuda.synchronize()
+            graph.replay()
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(result.cpu().numpy(), eager, err_msg=f"replay {rep}")
+
+
+def test_graph_capture_multi_grg_mixed_heights(artifact_dir, art_b):
+    """Two GRGs of different heights (4-level band, synthetic) share one runtime and
+    one captured graph (reference test_triton.py:978-1014)."""
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    band = write_band_artifact(artifact_dir, "band-h4-n16", n=16, bandwidth=4, levels=4)
+    arts = [band, art_b]
+    oracles = [O.load_artifact(p) for p in arts]
+    with torch.cuda.device(0):
+        cap = torch.cuda.Stream()
+    layout = build_hip_layout(arts, requirements=full_requirements(max_k_up=1, max_k_down=1), stream=cap)
+    orders = [("up", "down"), ("down", "up")]
+    cases = []
+    for i, (g, order) in enumerate(zip(oracles, orders)):
+        src = np.arange(_cols(g, order[0]), dtype=np.float64).reshape(1, -1) + 100.0 * i
+        i1, i2 = np.array([0.25 + i]), np.array([0.75 + i])
+        cases.append((order, src, i1, i2, _chain_expected(g, order, src, i1, i2)))
+    with HipRuntime(layout) as rt, ExitStack() as stack:
+        chains = [_enter_chain(stack, grg, case[0], 1) for grg, case in zip(rt.grgs, cases)]
+        dev = chains[0][0].input.device
+        tens = [(torch.from_numpy(c[1]).to(dev), torch.from_numpy(c[2]).to(dev), torch.from_numpy(c[3]).to(dev)) for c in cases]
+        results = [torch.empty_like(ch[1].output) for ch in chains]
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=cap):
+            for (first, second), (src, i1, i2), res in zip(chains, tens, results):
+                _run_chain(first, second, src, i1, i2, res)
+        for rep in range(2):
+            graph.replay()
+            torch.cuda.synchronize()
+            for case, res in zip(cases, results):
+                _check(res.cpu().numpy(), case[4])
+
+
+# ---------------------------------------------------------------------------
+# several GRGs, mixed k, concurrent runtimes, lifecycle
+# ---------------------------------------------------------------------------
+
+
+def test_multi_grg_mixed_k(artifact_dir, art_a, art_b):
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    band = write_band_artifact(artifact_dir, "band-n64", n=64, bandwidth=8)
+    arts = [art_a, art_b, band]
+    oracles = [O.load_artifact(p) for p in arts]
+    layout = build_hip_layout(arts, requirements=full_requirements(max_k_up=16, max_k_down=16))
+    with HipRuntime(layout) as rt:
+        assert len(rt.grgs) == 3
+        rng = np.random.default_rng(3800)
+        for grg, g, k in zip(rt.grgs, oracles, (1, 7, 16)):
+            for direction in ("up", "down"):
+                x = rng.standard_normal((k, _cols(g, direction)))
+                _check(grg.matmul(x, direction), O.matmul(g, x, direction))
+
+
+def test_two_runtimes_on_separate_streams(art_a, art_b):
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    with torch.cuda.device(0):
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    req = full_requirements(max_k_up=3, max_k_down=3)
+    oa, ob = O.load_artifact(art_a), O.load_artifact(art_b)
+    with HipRuntime(build_hip_layout([art_a], requirements=req, stream=s1)) as r1, \
+            HipRuntime(build_hip_layout([art_b], requirements=req, stream=s2)) as r2:
+        (ga,), (gb,) = r1.grgs, r2.grgs
+        rng = np.random.default_rng(3900)
+        xa = rng.standard_normal((3, oa.num_samples))
+        xb = rng.standard_normal((3, ob.num_samples))
+        with ExitStack() as stack:
+            pa = stack.enter_context(ga.prepare_matmul_cuda(direction="up", k=3))
+            pb = stack.enter_context(gb.prepare_matmul_cuda(direction="up", k=3))
+            with torch.cuda.stream(s1):
+                pa.input.copy_(torch.from_numpy(xa).to(pa.input.device))
+            with torch.cuda.stream(s2):
+                pb.input.copy_(torch.from_numpy(xb).to(pb.input.device))
+            # one call each: an op's output shares the arena with its input
+            pa()
+            pb()
+            torch.cuda.synchronize()
+            _check(pa.output.cpu().numpy(), O.matmul(oa, xa, "up"))
+            _check(pb.output.cpu().numpy(), O.matmul(ob, xb, "up"))
+
+
+def test_runtime_lifecycle(art_a):
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=2, max_k_down=2))
+    rt = HipRuntime(layout)
+    with pytest.raises(RuntimeError):
+        _ = rt.grgs
+    with rt:
+        (grg,) = rt.grgs
+        with rt._call_scope():
+            with pytest.raises(RuntimeError, match="concurrent runtime.grgs calls are not supported"):
+                with rt._call_scope():
+                    pass
+        up = grg.prepare_matmul_cuda(direction="up", k=2)
+        down = grg.prepare_matmul_cuda(direction="down", k=2)
+        assert up.output.data_ptr() == down.input.data_ptr()
+        assert down.output.data_ptr() == up.input.data_ptr()
+    with pytest.raises(RuntimeError):
+        _ = rt.grgs
+
+
+def test_disabled_direction_rejected(art_a):
+    from pygrgl_spmv.backends.hip import HipRuntime
+    from tests._builders import hip_pair
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=2, max_k_down=2), pair=hip_pair(down=False))
+    with HipRuntime(layout) as rt:
+        (grg,) = rt.grgs
+        with pytest.raises(ValueError, match="DOWN plan is not configured"):
+            grg.prepare_matmul_cuda(direction="down", k=1)

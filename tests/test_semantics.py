@@ -228,17 +228,23 @@ def test_ctypes_struct_layout_matches_c_header(tmp_path):
     """sizeof / offsetof of GrgHipGatherArgs from the C compiler equal ctypes' layout."""
     from pygrgl_spmv.backends.hip import ffi
 
+    structs = {"GrgHipGatherArgs": ffi.GrgHipGatherArgs, "GrgHipSeg": ffi.GrgHipSeg, "GrgHipMap": ffi.GrgHipMap}
+    exprs, want = [], []
+    for cname, cls in structs.items():
+        exprs.append(f"sizeof({cname})")
+        want.append(ctypes.sizeof(cls))
+        for fname, _ in cls._fields_:
+            exprs.append(f"offsetof({cname}, {fname})")
+            want.append(getattr(cls, fname).offset)
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "grg_hip.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(GrgHipGatherArgs), offsetof(GrgHipGatherArgs, seg),"
-        " offsetof(GrgHipGatherArgs, map), offsetof(GrgHipGatherArgs, heavy_rows), sizeof(GrgHipSeg)); return 0;}\n"
+        "int main(void){\n" + "".join(f'  printf("%zu\\n", (size_t)({e}));\n' for e in exprs) + "  return 0;}\n"
     )
     exe = tmp_path / "layout"
     subprocess.run(["gcc", f"-I{REPO / 'include'}", "-o", str(exe), str(src)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
-    A = ffi.GrgHipGatherArgs
-    assert got == [ctypes.sizeof(A), A.seg.offset, A.map.offset, A.heavy_rows.offset, ctypes.sizeof(ffi.GrgHipSeg)]
+    assert dict(zip(exprs, got)) == dict(zip(exprs, want))
 
 
 def test_hip_library_reports_errors_without_gpu():
