@@ -1,7 +1,9 @@
 """Build the native libraries in-tree.
 
-* ``pygrgl_spmv/_lib/libgrg_hip.so``   -- HIP kernels + C ABI (hipcc, gfx950)
+* ``pygrgl_spmv/_lib/libgrg_hip.so``   -- HIP sweep kernels + C ABI (hipcc, gfx950)
+* ``pygrgl_spmv/_lib/libgrg_mlp.so``   -- HIP warm-start network kernels (hipcc, gfx950)
 * ``pygrgl_spmv/_lib/libgrg_synth.so`` -- synthetic GRG generator (g++)
+* ``pygrgl_spmv/_lib/libgrg_host.so``  -- host-side node-order pass (g++)
 
 Run ``python grg-spmv_amd/build.py`` (or ``__graft_entry__.build()``).
 """
@@ -68,8 +70,15 @@ def build_host() -> Path:
     return target
 
 
+def build_mlp() -> Path:
+    LIB.mkdir(parents=True, exist_ok=True)
+    target = LIB / "libgrg_mlp.so"
+    _atomic_build(lambda out: [HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", "-o", str(out), str(CSRC / "grg_mlp.hip")], target)
+    return target
+
+
 def build_all() -> list[Path]:
-    return [build_hip(), build_synth(), build_host()]
+    return [build_hip(), build_mlp(), build_synth(), build_host()]
 
 
 if __name__ == "__main__":

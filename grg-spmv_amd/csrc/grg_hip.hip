@@ -218,8 +218,14 @@ __device__ __forceinline__ void block_reduce(Acc (&acc)[CPL], int KL, Acc (&part
 template <typename T, int CPL, typename Acc>
 __device__ __forceinline__ void store_row(const GrgHipGatherArgs& a, int64_t row, int col, const Acc (&acc)[CPL]) {
   T v[CPL];
+  if (a.out_scale) {
+    const double sc = (double)static_cast<const T*>(a.out_scale)[row - a.out_scale_base];
 #pragma unroll
-  for (int c = 0; c < CPL; ++c) v[c] = (T)acc[c].value();
+    for (int c = 0; c < CPL; ++c) v[c] = (T)(acc[c].wide() * sc);
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) v[c] = (T)acc[c].value();
+  }
   store_raw<T, CPL>(static_cast<T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
 }
 

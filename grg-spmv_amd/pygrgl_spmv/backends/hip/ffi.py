@@ -13,7 +13,7 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 F64 = 0
 F32C = 1
 FLAG_XCD_REMAP = 2
@@ -60,6 +60,8 @@ class GrgHipGatherArgs(ctypes.Structure):
         ("split_items", c_int64),
         ("split_chunk", c_int64),
         ("partials", c_void_p),
+        ("out_scale", c_void_p),
+        ("out_scale_base", c_int64),
     ]
 
 

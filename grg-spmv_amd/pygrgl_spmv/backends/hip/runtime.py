@@ -257,6 +257,31 @@ class _HipPreparedMatmul:
         eagerly only (capturing them is refused with a RuntimeError)."""
         return not any(it[0] == "stream" for it in self._steps)
 
+    def set_output_scale(self, scale) -> None:
+        """Multiply every output row by ``scale[row]`` inside the write-out launch
+        (no extra pass). For the up direction without emit_all_nodes or missing-data
+        output; ``scale`` is a device tensor of ``output_cols`` values in the
+        runtime's dtype, kept alive by the op. ``None`` removes the scale. Used by
+        the standardized-genotype operator (no reference counterpart)."""
+        spec = self._spec
+        if spec.direction != Direction.UP or spec.emit_all_nodes or spec.use_miss:
+            raise ValueError("set_output_scale() applies to up ops without emit_all_nodes or missing-data output")
+        kind, arr, n = self._steps[-1]
+        if kind != "batch":
+            raise RuntimeError("internal error: the write-out is not the last batched launch")
+        if scale is None:
+            arr[n - 1].out_scale, arr[n - 1].out_scale_base = 0, 0
+            self._out_scale = None
+            return
+        torch = _torch()
+        if not isinstance(scale, torch.Tensor) or scale.device != self._runtime.device:
+            raise TypeError("scale must be a device tensor on the runtime's device")
+        if scale.dtype != self._runtime._torch_dtype() or scale.dim() != 1 or scale.numel() != int(spec.output_cols):
+            raise ValueError(f"scale must be a 1-D {self._runtime._torch_dtype()} tensor of {int(spec.output_cols)} values")
+        scale = scale.contiguous()
+        arr[n - 1].out_scale, arr[n - 1].out_scale_base = scale.data_ptr(), 0
+        self._out_scale = scale
+
     def __call__(self) -> None:
         with self._runtime._call_scope():
             self._call_prelocked()

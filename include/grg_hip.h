@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 4
+#define GRG_HIP_ABI_VERSION 5
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -112,6 +112,11 @@ typedef struct GrgHipGatherArgs {
   int64_t split_items;
   int64_t split_chunk;        /* segment entries per item */
   void* partials;             /* double, split_items x k */
+  /* Optional output row scale (no reference counterpart; used by the
+   * standardized-genotype operator): every produced row is multiplied by
+   * out_scale[row - out_scale_base] after all its terms are summed. */
+  const void* out_scale;
+  int64_t out_scale_base;
 } GrgHipGatherArgs;
 
 /* One row-gather launch (see above). Replaces one level of the reference's
