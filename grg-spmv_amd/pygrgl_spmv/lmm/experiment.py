@@ -10,9 +10,11 @@ Steps (all on the GPU except the artifact generation):
 3. ``n_train`` synthetic phenotypes solved by plain zero-start CG to the
    tolerance: the network's training set;
 4. the warm-start network trained on them (HIP forward / backward / Adam);
-5. ``n_test`` fresh phenotypes solved three ways to the same residual
-   (||r|| <= tol ||P y||): from zero, from the closed-form eigenvalue start, and
-   from the network's start; iterations per phenotype are compared.
+5. ``n_test`` fresh phenotypes solved to the same residual (||r|| <= tol ||P y||)
+   from zero, from the closed-form eigenvalue start and from the network's
+   start; iterations per phenotype are compared. A fourth solve starts from the
+   best coefficients of the same form (fitted to the zero-start solution): the
+   bound any predictor of those coefficients can reach on this data.
 
 Prints progress lines on stderr and one JSON line on stdout.
 """
@@ -33,7 +35,7 @@ def log(msg: str) -> None:
 
 
 def solve_batches(op, phen, *, rel_tol, max_iter, start=None):
-    """Solve every k-column batch of ``phen``; ``start(y, delta)`` gives x0 (None: zero start)."""
+    """Solve every k-column batch of ``phen``; ``start(y, delta, first_column)`` gives x0 (None: zero start)."""
     import torch
 
     from pygrgl_spmv.lmm.cg import conj_grad, shifted
@@ -42,7 +44,7 @@ def solve_batches(op, phen, *, rel_tol, max_iter, start=None):
     for s in range(0, phen.y.shape[1], op.k):
         y = phen.y[:, s:s + op.k]
         d = phen.delta[s:s + op.k]
-        x0 = None if start is None else start(y, d)
+        x0 = None if start is None else start(y, d, s)
         res = conj_grad(shifted(op, d), y, x0=x0, rel_tol=rel_tol, max_iter=max_iter)
         xs.append(res.x)
         cols.append(res.column_iterations)
@@ -61,8 +63,7 @@ def run(artifacts, *, k: int = 64, r: int = 20, n_train: int = 2048, n_test: int
     from pygrgl_spmv.lmm.operator import StandardizedGrgOperator
     from pygrgl_spmv.lmm.pcs import top_eigenpairs
     from pygrgl_spmv.lmm.simulate import simulate
-    from pygrgl_spmv.lmm.warmstart import (Basis, WarmStartNet, closed_form_coefficients, features,
-                                           start_from_coefficients, targets)
+    from pygrgl_spmv.lmm.warmstart import Basis, WarmStartNet, closed_form_coefficients, start_from_coefficients, targets
 
     req = RuntimeRequirements(max_k_up=k, max_k_down=k, need_down_miss_input=False, need_up_miss_output=False,
                               need_init_vector=False, need_init_matrix=False, need_init_xtx=True)
@@ -94,7 +95,7 @@ def run(artifacts, *, k: int = 64, r: int = 20, n_train: int = 2048, n_test: int
 
         t = time.perf_counter()
         net = WarmStartNet(basis, seed=seed)
-        history = net.fit(features(basis, train.y, train.delta), targets(basis, train.y, x_train), epochs=epochs)
+        history = net.fit(train.y, train.delta, x_train, epochs=epochs)
         torch.cuda.synchronize()
         out["train_seconds"] = time.perf_counter() - t
         out["train_loss_first_last"] = [history[0], history[-1]]
@@ -104,8 +105,12 @@ def run(artifacts, *, k: int = 64, r: int = 20, n_train: int = 2048, n_test: int
         results = {}
         starts = {
             "zero": None,
-            "closed_form": lambda y, d: start_from_coefficients(basis, y, closed_form_coefficients(basis, y, d)),
-            "network": net.start,
+            "closed_form": lambda y, d, s: start_from_coefficients(basis, y, closed_form_coefficients(basis, y, d)),
+            "network": lambda y, d, s: net.start(y, d),
+            # the least-squares best start of the same form, fitted to the zero-start solution:
+            # the bound any predictor of those coefficients can reach on this data
+            "best_in_span": lambda y, d, s: start_from_coefficients(
+                basis, y, targets(basis, y, xs["zero"][:, s:s + y.shape[1]])),
         }
         xs = {}
         for name, start in starts.items():
@@ -118,7 +123,7 @@ def run(artifacts, *, k: int = 64, r: int = 20, n_train: int = 2048, n_test: int
                              "max_rel_residual": float(rels.max()), "seconds": time.perf_counter() - t,
                              "unconverged": int((its < 0).sum())}
             log(f"test {name}: mean {its.mean():.1f} iterations, {apps} operator applications")
-        for name in ("closed_form", "network"):
+        for name in ("closed_form", "network", "best_in_span"):
             diff = torch.linalg.norm(xs[name] - xs["zero"], dim=0) / torch.linalg.norm(xs["zero"], dim=0)
             results[name]["max_rel_diff_vs_zero_start"] = float(diff.max())
             results[name]["iterations_saved"] = 1.0 - results[name]["mean_iterations"] / results["zero"]["mean_iterations"]

@@ -15,8 +15,10 @@ everything is normalised by ||P y||:
 
 ``closed_form_coefficients`` is the same start built from the eigenvalues
 alone (c_i = (u_i^T y_hat) / (lambda_i + delta), a = 1 / (lambda_rest + delta) with
-lambda_rest the mean of K's remaining eigenvalues); it is reported beside the
-network as a second baseline. The network is ``mlp.DenseNet`` (HIP kernels).
+lambda_rest the mean of K's remaining eigenvalues). The network
+(``mlp.DenseNet``, HIP kernels) learns the correction from these closed-form
+coefficients to the best ones; the closed form alone is also reported as a
+second baseline.
 """
 
 from __future__ import annotations
@@ -92,8 +94,11 @@ def closed_form_coefficients(basis: Basis, y, delta):
 
 
 class WarmStartNet:
-    """Inputs and targets standardised with training-set statistics, then
-    ``DenseNet([r + 1, *hidden, r + 1])`` trained with Adam on the GPU."""
+    """The network predicts a correction to the closed-form coefficients: its output
+    (de-standardised with training-set statistics) is added to
+    ``closed_form_coefficients``, so an untrained or useless network falls back to
+    the closed-form start. ``DenseNet([r + 1, *hidden, r + 1])``, trained with Adam
+    on the GPU."""
 
     def __init__(self, basis: Basis, *, hidden=(256, 256), seed: int = 0):
         self.basis = basis
@@ -101,18 +106,28 @@ class WarmStartNet:
         self.net = DenseNet([r + 1, *hidden, r + 1], device=basis.U.device, seed=seed)
         self.stats = None
 
-    def fit(self, F, T, *, epochs: int = 300, batch: int = 128, lr: float = 1e-3, seed: int = 0):
+    def fit(self, y, delta, x_star, *, epochs: int = 300, batch: int = 128, lr: float = 1e-3, seed: int = 0):
+        """Train on solutions ``x_star`` of the plain method for phenotypes ``y`` (n x c)
+        with shifts ``delta`` (c); returns the per-epoch training loss."""
+        import torch
+
+        F = features(self.basis, y, delta)
+        T = targets(self.basis, y, x_star) - closed_form_coefficients(self.basis, y, delta).to(torch.float32)
         fm, fs = F.mean(dim=0), F.std(dim=0) + 1e-6
         tm, ts = T.mean(dim=0), T.std(dim=0) + 1e-12
         self.stats = (fm, fs, tm, ts)
         return self.net.train((F - fm) / fs, (T - tm) / ts, epochs=epochs, batch=batch, lr=lr, seed=seed)
 
-    def coefficients(self, F):
+    def coefficients(self, y, delta):
+        import torch
+
         fm, fs, tm, ts = self.stats
-        return self.net.forward((F - fm) / fs) * ts + tm
+        F = features(self.basis, y, delta)
+        correction = self.net.forward((F - fm) / fs) * ts + tm
+        return closed_form_coefficients(self.basis, y, delta) + correction.to(torch.float64)
 
     def start(self, y, delta):
-        return start_from_coefficients(self.basis, y, self.coefficients(features(self.basis, y, delta)))
+        return start_from_coefficients(self.basis, y, self.coefficients(y, delta))
 
 
 __all__ = ["Basis", "WarmStartNet", "closed_form_coefficients", "features", "start_from_coefficients", "targets"]
