@@ -12,7 +12,8 @@
 // many independent gathers in flight with few registers:
 //
 // Light path. A wave of 64 lanes is split into lane groups of G = kl * es
-// lanes, one row per group; kl lanes cover a chunk of kl * cpl columns with
+// lanes, R = 1, 2 or 4 consecutive rows per group (the R rows' gathers share
+// each round, so short rows overlap their latency); kl lanes cover a chunk of kl * cpl columns with
 // cpl-wide vector loads, es lanes interleave over the row's entries (so the
 // group's lanes read adjacent entries, which the locality order of the node
 // ids keeps close in memory). Each lane works in rounds of U (4, 8 or 16)
@@ -160,6 +161,49 @@ __device__ __forceinline__ void gather_entries(const int32_t* __restrict__ idx, 
   }
 }
 
+// R rows at once: the same rounds of U predicated gathers, for R independent
+// entry ranges [j[t], end[t]) side by side, so one round trip serves R short
+// rows. Rounds continue while any of the R ranges has entries left.
+template <typename T, int CPL, int U, int R, typename Acc>
+__device__ __forceinline__ void gather_rows(const int32_t* __restrict__ idx, const T* __restrict__ src, int64_t ld,
+                                            int32_t (&j)[R], const int32_t (&end)[R], int32_t stride,
+                                            Acc (&acc)[R][CPL]) {
+  while (true) {
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) any |= j[t] < end[t];
+    if (!any) break;
+    int32_t ii[R][U];
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t jt = j[t] + u * stride;
+        ii[t][u] = jt < end[t] ? idx[jt] : -1;
+      }
+    T v[R][U][CPL];
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (ii[t][u] >= 0) {
+          load_raw<T, CPL>(src + (int64_t)ii[t][u] * ld, v[t][u]);
+        } else {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) v[t][u][c] = T(0);
+        }
+      }
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[t][c].add(v[t][u][c]);
+      j[t] += U * stride;
+    }
+  }
+}
+
 // Row-wise seed / epilogue terms (added once per row by one lane).
 template <typename T, int CPL, typename Acc>
 __device__ __forceinline__ void add_row_terms(const GrgHipGatherArgs& a, int64_t row, int col, Acc (&acc)[CPL]) {
@@ -229,7 +273,7 @@ __device__ __forceinline__ void store_row(const GrgHipGatherArgs& a, int64_t row
   store_raw<T, CPL>(static_cast<T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
 }
 
-template <typename T, int CPL, int U>
+template <typename T, int CPL, int U, int R>
 __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherArgs a, int64_t n_light_blocks,
                                                              int g_shift, int kl_shift) {
   using Acc = typename SumOf<T>::type;
@@ -276,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
   }
 
   if ((int64_t)blockIdx.x < n_light_blocks) {
-    // ---- light path: one row per G-lane group ----
+    // ---- light path: R consecutive rows per G-lane group ----
     const int G = 1 << g_shift;
     const int lg = lane & (G - 1);
     const int e = lg >> kl_shift;
@@ -288,30 +332,64 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
       const int64_t q = n_light_blocks >> 3, r = n_light_blocks & 7, x = blk & 7;
       blk = x * q + (x < r ? x : r) + (blk >> 3);
     }
-    const int64_t row = a.row_begin + (((blk * kWavesPerBlock + wave) << (6 - g_shift)) + (lane >> g_shift));
-    bool live = row < a.row_end && col < a.k;
-    if (live) {
-      int32_t p0[GRG_HIP_MAX_SEGS], p1[GRG_HIP_MAX_SEGS];
-      int64_t deg = 0;
-      for (int s = 0; s < a.nseg; ++s) {
-        const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
-        p0[s] = ptr[0];
-        p1[s] = ptr[1];
-        deg += p1[s] - p0[s];
+    const int64_t row0 = a.row_begin + (int64_t)R * (((blk * kWavesPerBlock + wave) << (6 - g_shift)) + (lane >> g_shift));
+    const bool any_live = row0 < a.row_end && col < a.k;
+    bool live[R];
+    Acc racc[R][CPL];
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      live[t] = false;
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) racc[t][c].zero();
+    }
+    if (any_live) {
+      // row pointers of the R rows in every segment (rows past row_end read the level's end)
+      int32_t pb[GRG_HIP_MAX_SEGS][R + 1];
+      int64_t deg[R];
+#pragma unroll
+      for (int t = 0; t < R; ++t) deg[t] = 0;
+#pragma unroll
+      for (int s = 0; s < GRG_HIP_MAX_SEGS; ++s) {
+        if (s < a.nseg) {
+          const int32_t* ptr = a.seg[s].ptr - a.seg[s].ptr_base;
+#pragma unroll
+          for (int t = 0; t <= R; ++t) pb[s][t] = ptr[row0 + t < a.row_end ? row0 + t : a.row_end];
+#pragma unroll
+          for (int t = 0; t < R; ++t) deg[t] += pb[s][t + 1] - pb[s][t];
+        }
       }
-      if (a.heavy_threshold >= 0 && deg > a.heavy_threshold) {
-        live = false;  // produced by the heavy path
-      } else {
-        for (int s = 0; s < a.nseg; ++s)
-          gather_entries<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, p0[s] + e, p1[s], ES, acc);
-        if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
+#pragma unroll
+      for (int t = 0; t < R; ++t)
+        live[t] = row0 + t < a.row_end && !(a.heavy_threshold >= 0 && deg[t] > a.heavy_threshold);  // heavy: heavy path
+#pragma unroll
+      for (int s = 0; s < GRG_HIP_MAX_SEGS; ++s) {
+        if (s < a.nseg) {
+          int32_t j[R], end[R];
+#pragma unroll
+          for (int t = 0; t < R; ++t) {
+            j[t] = pb[s][t] + e;
+            end[t] = live[t] ? pb[s][t + 1] : 0;
+          }
+          gather_rows<T, CPL, U, R>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, j, end, ES, racc);
+        }
+      }
+      if (e == 0) {
+#pragma unroll
+        for (int t = 0; t < R; ++t)
+          if (live[t]) add_row_terms<T, CPL>(a, row0 + t, col, racc[t]);
       }
     }
     for (int off = KL; off < G; off <<= 1) {
 #pragma unroll
-      for (int c = 0; c < CPL; ++c) acc[c].merge_xor(off);
+      for (int t = 0; t < R; ++t)
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) racc[t][c].merge_xor(off);
     }
-    if (live && e == 0) store_row<T, CPL>(a, row, col, acc);
+    if (any_live && e == 0) {
+#pragma unroll
+      for (int t = 0; t < R; ++t)
+        if (live[t]) store_row<T, CPL>(a, row0 + t, col, racc[t]);
+    }
     return;
   }
 
@@ -364,7 +442,12 @@ int validate(const GrgHipGatherArgs& a) {
   if (a.cpl != 1 && a.cpl != 2 && a.cpl != 4) return fail("cpl must be 1, 2 or 4");
   if (!is_pow2(a.kl) || !is_pow2(a.es) || a.kl * a.es > 64) return fail("kl and es must be powers of two with kl * es <= 64");
   if (a.kl * a.cpl > 64) return fail("kl * cpl must be <= 64");
-  if (a.unroll != 0 && a.unroll != 4 && a.unroll != 8 && a.unroll != 16) return fail("unroll must be 0, 4, 8 or 16");
+  {
+    const int u = a.unroll ? a.unroll : 8;
+    const int r = a.rows_per_group ? a.rows_per_group : 1;
+    const bool ok = (r == 1 && (u == 4 || u == 8 || u == 16)) || (r == 2 && (u == 4 || u == 8)) || (r == 4 && u == 4);
+    if (!ok) return fail("(unroll, rows_per_group) must be one of (4|8|16, 1), (4|8, 2), (4, 4)");
+  }
   if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS)) return fail("unknown flags");
   if (a.k % a.cpl) return fail("k must be a multiple of cpl");
   if (!a.out) return fail("out is null");
@@ -393,18 +476,22 @@ int validate(const GrgHipGatherArgs& a) {
   return 0;
 }
 
-template <typename T, int CPL, int U>
+template <typename T, int CPL, int U, int R>
 void launch_typed(const GrgHipGatherArgs& a, dim3 grid, int64_t n_light, int g_shift, int kl_shift, hipStream_t s) {
-  hipLaunchKernelGGL((row_gather_kernel<T, CPL, U>), grid, dim3(kBlock), 0, s, a, n_light, g_shift, kl_shift);
+  hipLaunchKernelGGL((row_gather_kernel<T, CPL, U, R>), grid, dim3(kBlock), 0, s, a, n_light, g_shift, kl_shift);
 }
 
+// (unroll, rows per group) pairs with at most 16 gathers in flight per lane
 template <typename T, int CPL>
 void launch_unrolled(const GrgHipGatherArgs& a, dim3 grid, int64_t n_light, int g_shift, int kl_shift, hipStream_t s) {
-  switch (a.unroll) {
-    case 4: launch_typed<T, CPL, 4>(a, grid, n_light, g_shift, kl_shift, s); break;
-    case 16: launch_typed<T, CPL, 16>(a, grid, n_light, g_shift, kl_shift, s); break;
-    default: launch_typed<T, CPL, 8>(a, grid, n_light, g_shift, kl_shift, s); break;
-  }
+  const int u = a.unroll ? a.unroll : 8;
+  const int r = a.rows_per_group ? a.rows_per_group : 1;
+  if (u == 4 && r == 1) launch_typed<T, CPL, 4, 1>(a, grid, n_light, g_shift, kl_shift, s);
+  else if (u == 16 && r == 1) launch_typed<T, CPL, 16, 1>(a, grid, n_light, g_shift, kl_shift, s);
+  else if (u == 4 && r == 2) launch_typed<T, CPL, 4, 2>(a, grid, n_light, g_shift, kl_shift, s);
+  else if (u == 8 && r == 2) launch_typed<T, CPL, 8, 2>(a, grid, n_light, g_shift, kl_shift, s);
+  else if (u == 4 && r == 4) launch_typed<T, CPL, 4, 4>(a, grid, n_light, g_shift, kl_shift, s);
+  else launch_typed<T, CPL, 8, 1>(a, grid, n_light, g_shift, kl_shift, s);
 }
 
 int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
@@ -415,7 +502,7 @@ int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
   if (a.flags & GRG_HIP_FLAG_PARTIALS) {
     gx = a.split_items;
   } else {
-    const int64_t rows_per_block = (int64_t)kWavesPerBlock * (64 / G);
+    const int64_t rows_per_block = (int64_t)kWavesPerBlock * (64 / G) * (a.rows_per_group ? a.rows_per_group : 1);
     n_light = (a.row_end - a.row_begin + rows_per_block - 1) / rows_per_block;
     gx = n_light + a.num_heavy;
   }

@@ -13,7 +13,7 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 F64 = 0
 F32C = 1
 FLAG_XCD_REMAP = 2
@@ -41,6 +41,7 @@ class GrgHipGatherArgs(ctypes.Structure):
         ("kl", c_int32),
         ("es", c_int32),
         ("unroll", c_int32),
+        ("rows_per_group", c_int32),
         ("flags", c_int32),
         ("out", c_void_p),
         ("ld_out", c_int64),

@@ -294,12 +294,14 @@ class HipRuntime:
     """Runtime-owned HIP execution with resident or streamed sparse blocks."""
 
     def __init__(self, layout: HipLayout, *, autotune: bool = True, split_min: int = SPLIT_MIN,
-                 split_chunk: int = SPLIT_CHUNK) -> None:
+                 split_chunk: int = SPLIT_CHUNK, fixed_shape: dict | None = None) -> None:
         """``autotune``: time the admissible launch shapes of every level launch when an
         op is prepared (otherwise a fixed default shape). ``split_min`` /
         ``split_chunk``: rows of gather degree above ``split_min`` are summed in
         chunks of ``split_chunk`` entries by separate workgroups (the layout's
-        scratch bound assumes the defaults)."""
+        scratch bound assumes the defaults). ``fixed_shape``: launch fields
+        (``unroll``, ``rows_per_group``, ``es``, ``flags``) forced on every launch
+        instead of tuning, for tests of each kernel variant."""
         if int(split_min) < 1 or int(split_chunk) < 1:
             raise ValueError("split_min and split_chunk must be >= 1")
         torch = _torch()
@@ -309,7 +311,11 @@ class HipRuntime:
         self.layout = layout
         self.device = torch.device("cuda", int(layout.device))
         self.stream_ptr = int(layout.stream_ptr)
-        self.autotune = bool(autotune)
+        self.autotune = bool(autotune) and not fixed_shape
+        allowed = {"unroll", "rows_per_group", "es", "flags"}
+        if fixed_shape and set(fixed_shape) - allowed:
+            raise ValueError(f"fixed_shape keys must be among {sorted(allowed)}")
+        self.fixed_shape = dict(fixed_shape or {})
         self.split_min = int(split_min)
         self.split_chunk = int(split_chunk)
         self._dtype_code = ffi.F64 if np.dtype(layout.dtype) == np.float64 else ffi.F32C
@@ -675,8 +681,13 @@ class HipRuntime:
             a.heavy_threshold, a.heavy_rows, a.num_heavy = -1, 0, 0
         a.flags = 0
         a.unroll = KERNEL_UNROLL
+        a.rows_per_group = 1
         lds = [ld_out] + [s[3] for s in segs] + [m[2] for m in maps]
         a.cpl, a.kl, a.es = launch_shape(self._dtype_code, int(k), lds, mean_degree)
+        for name, value in self.fixed_shape.items():
+            setattr(a, name, int(value))
+        if a.kl * a.es > 64:
+            a.es = 64 // a.kl
         a.out, a.ld_out, a.out_base = int(out), int(ld_out), int(out_base)
         a.nseg = len(segs)
         for i, (ptr, idx, src, ld, base) in enumerate(segs):
@@ -706,9 +717,10 @@ class HipRuntime:
     def _tune(self, a: ffi.GrgHipGatherArgs, key) -> None:
         """Per-level autotuner, run when an op is prepared (never in ``op()``; the
         timing launches write only the level's node rows). Stage 1 times every
-        admissible (cpl, kl, es) shape at 8 gathers in flight per lane; stage 2
-        tries 4 and 16 gathers in flight and the XCD remap on the best shape. The
-        fastest launch is cached per artifact, sweep configuration, level and k."""
+        admissible (cpl, kl, es) shape with 8 gathers in flight and one row per lane
+        group; stage 2 tries the other (gathers in flight, rows per group) pairs and
+        the XCD remap on the best shape. The fastest launch is cached per artifact,
+        sweep configuration, level and k."""
         torch = _torch()
         best = self._tune_cache.get(key)
         if best is None and self.autotune and a.row_end > a.row_begin:
@@ -717,26 +729,27 @@ class HipRuntime:
             ctypes.pointer(trial)[0] = a
             elem = 8 if a.dtype == ffi.F64 else 4
             timings = []
+
+            def time_shape(cpl, kl, es, unroll, rows, fl):
+                trial.cpl, trial.kl, trial.es, trial.unroll, trial.rows_per_group, trial.flags = cpl, kl, es, unroll, rows, fl
+                timings.append((self._time_launch(trial), cpl, kl, es, unroll, rows, fl))
+
             with torch.cuda.device(self.device):
                 for cpl, kl in column_shapes(a.dtype, a.k, lds):
                     for es in (1, 2, 4, 8, 16, 32, 64):
-                        if kl * es > 64:
-                            continue
-                        trial.cpl, trial.kl, trial.es, trial.unroll, trial.flags = cpl, kl, es, 8, 0
-                        timings.append((self._time_launch(trial), cpl, kl, es, 8, 0))
-                _, cpl, kl, es, _, _ = min(timings)
-                for unroll in (4, 8, 16):
-                    if unroll * cpl * elem > 256:  # keep the loads in flight within 64 registers
+                        if kl * es <= 64:
+                            time_shape(cpl, kl, es, 8, 1, 0)
+                _, cpl, kl, es, _, _, _ = min(timings)
+                for unroll, rows in ((4, 1), (16, 1), (4, 2), (8, 2), (4, 4)):
+                    if unroll * rows * cpl * elem > 256:  # keep the loads in flight within 64 registers
                         continue
-                    for fl in (0, ffi.FLAG_XCD_REMAP):
-                        if (unroll, fl) == (8, 0):
-                            continue
-                        trial.cpl, trial.kl, trial.es, trial.unroll, trial.flags = cpl, kl, es, unroll, fl
-                        timings.append((self._time_launch(trial), cpl, kl, es, unroll, fl))
+                    time_shape(cpl, kl, es, unroll, rows, 0)
+                _, cpl, kl, es, unroll, rows, _ = min(timings)
+                time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_XCD_REMAP)
             best = min(timings)[1:]
             self._tune_cache[key] = best
         if best is not None:
-            a.cpl, a.kl, a.es, a.unroll, a.flags = best
+            a.cpl, a.kl, a.es, a.unroll, a.rows_per_group, a.flags = best
 
     def _build_steps(self, art: _HipArtifact, spec: _CudaMatmulSpec):
         """The prepared sweep: a list of ("batch", array, n) and ("stream", step, args, events) items."""

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 5
+#define GRG_HIP_ABI_VERSION 6
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -83,7 +83,9 @@ typedef struct GrgHipGatherArgs {
   int32_t cpl;          /* columns per lane: 1, 2 or 4; k, every ld and every base must be multiples */
   int32_t kl;           /* lanes across one column chunk (power of two, kl * cpl columns per chunk) */
   int32_t es;           /* lanes interleaving over one row's entries (power of two, kl * es <= 64) */
-  int32_t unroll;       /* gathers in flight per lane and round: 4, 8 or 16 (0 means 8) */
+  int32_t unroll;       /* gathers in flight per lane, row and round: 4, 8 or 16 (0 means 8) */
+  int32_t rows_per_group; /* consecutive rows per lane group: 1, 2 or 4 (0 means 1); with unroll:
+                             (4|8|16, 1), (4|8, 2) or (4, 4) */
   int32_t flags;        /* GRG_HIP_FLAG_* launch options */
   void* out;            /* out row r lives at out + (r - out_base) * ld_out */
   int64_t ld_out;

@@ -114,6 +114,29 @@ def test_untuned_launch_shapes(art_a, oracle_a, k):
             _check(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
 
 
+@pytest.mark.parametrize("unroll,rows", [(4, 1), (8, 1), (16, 1), (4, 2), (8, 2), (4, 4)])
+@pytest.mark.parametrize("es", [1, 4])
+def test_every_kernel_variant(art_a, oracle_a, unroll, rows, es):
+    """Each (gathers in flight, rows per lane group) variant of the light path, with the
+    XCD remap, against the oracle in every sweep configuration."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+    from pygrgl_spmv.backends.hip import ffi
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=8, max_k_down=8))
+    shape = {"unroll": unroll, "rows_per_group": rows, "es": es, "flags": ffi.FLAG_XCD_REMAP}
+    with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16) as rt:
+        (grg,) = rt.grgs
+        rng = np.random.default_rng(3350 + 10 * unroll + rows + es)
+        for direction in ("up", "down"):
+            x = rng.choice(np.array([-1.0, 1.0]), size=(5, _cols(oracle_a, direction)))
+            np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
+        y = rng.standard_normal((3, oracle_a.num_mutations))
+        miss = rng.standard_normal((3, oracle_a.num_mutations))
+        _check(grg.matmul(y, "down", miss=miss), O.matmul(oracle_a, y, "down", miss=miss))
+        x = rng.standard_normal((8, oracle_a.num_individuals))
+        _check(grg.matmul(x, "up", by_individual=True, init="xtx"), O.matmul(oracle_a, x, "up", by_individual=True, init="xtx"))
+
+
 # ---------------------------------------------------------------------------
 # streamed blocks through the pinned ring
 # ---------------------------------------------------------------------------
