@@ -1,0 +1,158 @@
+"""Several GRGs (one per chromosome) in one HIP runtime, captured into one hipGraph.
+
+    python -m pygrgl_spmv.chromosomes [--chromosomes 22 --samples 200000 --budget-gb 32 --ring 4]
+
+The batched configuration of the benchmark suite: per-chromosome synthetic GRGs
+over the same samples with uneven mutation counts (U[20k, 400k]), one runtime
+under a VRAM budget with a ring of streaming slots (``runtime.grgs`` as in the
+reference's multi-GRG runtime, ``triton/backend.py:373-410``), mixed column
+counts k in [1, 64] per GRG. Every GRG's up and down op (input copy, sweep,
+output copy) is recorded into ONE hipGraph when every block is resident; the
+graph is replayed and each replay checked bit for bit against the eager run.
+(Parity of multi-GRG runtimes with the CPU oracle is covered by the tests.)
+
+Prints progress on stderr and one JSON line on stdout.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+
+def log(msg: str) -> None:
+    print(f"[chromosomes] {msg}", file=sys.stderr, flush=True)
+
+
+def mixed_k(i: int) -> int:
+    """Column counts cycling through 1..64 (powers of two and odd sizes)."""
+    return (1, 3, 8, 16, 5, 32, 64, 2, 12, 48, 7)[i % 11]
+
+
+def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: int = 3, seed: int = 4) -> dict:
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipPlanPair, HipRuntime, plan_hip_layout
+    from pygrgl_spmv.grg import RuntimeRequirements
+
+    req = RuntimeRequirements(max_k_up=64, max_k_down=64, need_down_miss_input=False, need_up_miss_output=False,
+                              need_init_vector=False, need_init_matrix=False, need_init_xtx=False)
+    with torch.cuda.device(0):
+        cap = torch.cuda.Stream()
+    t0 = time.perf_counter()
+    layout = plan_hip_layout(artifacts=[Path(a) for a in artifacts], pair=HipPlanPair.default(), dtype=dtype, requirements=req,
+                             vram_budget_bytes=int(budget_bytes), ring_buffer_size=int(ring), device=0, stream=cap)
+    blocks = [b for a in layout.artifacts for b in (*a.blocks_up, *a.blocks_down)]
+    out = {"grgs": len(artifacts), "budget_bytes": int(budget_bytes), "bytes_total": int(layout.bytes_total),
+           "required_for_full_residency": int(layout.required_budget_for_full_residency),
+           "blocks": len(blocks), "streamed_blocks": sum(not b.resident for b in blocks),
+           "ring_slots": int(layout.allocated_ring_buffer_size)}
+    log(f"layout: {out['bytes_total'] / 1e9:.2f} GB of {budget_bytes / 1e9:.1f} GB budget, "
+        f"{out['streamed_blocks']} of {out['blocks']} blocks streamed")
+    rng = np.random.default_rng(seed)
+    with HipRuntime(layout) as rt:
+        grgs = rt.grgs
+        ops, xs, ys, res_up, res_dn = [], [], [], [], []
+        edges = 0
+        for i, g in enumerate(grgs):
+            k = mixed_k(i)
+            up = g.prepare_matmul_cuda(direction="up", k=k)
+            dn = g.prepare_matmul_cuda(direction="down", k=k)
+            ops.append((up, dn))
+            xs.append(torch.from_numpy(rng.standard_normal((k, g.num_samples))).to(rt.device, up.input.dtype))
+            ys.append(torch.from_numpy(rng.standard_normal((k, g.num_mutations))).to(rt.device, dn.input.dtype))
+            res_up.append(torch.empty_like(up.output))
+            res_dn.append(torch.empty_like(dn.output))
+            edges += 2 * int(g.num_edges) * k
+        out["ks"] = [mixed_k(i) for i in range(len(grgs))]
+        out["edge_updates_per_pass"] = edges
+        out["edges"] = [int(g.num_edges) for g in grgs]
+        log(f"runtime entered and {2 * len(grgs)} ops prepared in {time.perf_counter() - t0:.1f}s")
+
+        def all_ops():
+            for (up, dn), x, y, ru, rd in zip(ops, xs, ys, res_up, res_dn):
+                up.input.copy_(x)
+                up()
+                ru.copy_(up.output)
+                dn.input.copy_(y)
+                dn()
+                rd.copy_(dn.output)
+
+        with torch.cuda.stream(cap):
+            all_ops()  # warm
+        torch.cuda.synchronize()
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        with torch.cuda.stream(cap):
+            ev[0].record(cap)
+            all_ops()
+            ev[1].record(cap)
+        torch.cuda.synchronize()
+        out["eager_ms"] = ev[0].elapsed_time(ev[1])
+        out["eager_edge_updates_per_s"] = edges / (out["eager_ms"] / 1e3)
+        eager_up = [r.cpu().clone() for r in res_up]
+        eager_dn = [r.cpu().clone() for r in res_dn]
+        capturable = all(up.capturable and dn.capturable for up, dn in ops)
+        out["captured"] = bool(capturable)
+        if capturable:
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, stream=cap):
+                all_ops()
+            times = []
+            for rep in range(int(replays)):
+                for r in (*res_up, *res_dn):
+                    r.zero_()
+                torch.cuda.synchronize()
+                with torch.cuda.stream(cap):
+                    ev[0].record(cap)
+                    graph.replay()
+                    ev[1].record(cap)
+                torch.cuda.synchronize()
+                times.append(ev[0].elapsed_time(ev[1]))
+                same = all(torch.equal(a.cpu(), b) for a, b in zip(res_up, eager_up)) and \
+                    all(torch.equal(a.cpu(), b) for a, b in zip(res_dn, eager_dn))
+                if not same:
+                    raise AssertionError(f"graph replay {rep} differs from the eager run")
+            out["graph_replay_ms"] = min(times)
+            out["graph_replays_checked"] = int(replays)
+            out["graph_edge_updates_per_s"] = edges / (min(times) / 1e3)
+        log(f"eager {out['eager_ms']:.2f} ms" + (f", graph replay {out['graph_replay_ms']:.2f} ms" if capturable else
+                                                  ", not captured (streamed blocks)"))
+    out["seconds_total"] = time.perf_counter() - t0
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--chromosomes", type=int, default=22)
+    ap.add_argument("--samples", type=int, default=200_000)
+    ap.add_argument("--budget-gb", type=float, default=32.0)
+    ap.add_argument("--ring", type=int, default=4)
+    ap.add_argument("--artifact-dir", default="/tmp/grg_chrom")
+    ap.add_argument("--fp32c", action="store_true")
+    args = ap.parse_args()
+
+    from pygrgl_spmv.synth import chromosome_workloads, write_artifact
+
+    d = Path(args.artifact_dir)
+    d.mkdir(parents=True, exist_ok=True)
+    arts = []
+    t = time.perf_counter()
+    for i, cfg in enumerate(chromosome_workloads(args.chromosomes, args.samples)):
+        p = d / f"chr{i + 1}-{args.samples}-seed{cfg.seed}.grg_spmv"
+        if not p.exists():
+            write_artifact(cfg, p)
+        arts.append(p)
+    log(f"{len(arts)} artifacts ready in {time.perf_counter() - t:.1f}s")
+    res = run(arts, budget_bytes=int(args.budget_gb * 1e9), ring=args.ring,
+              dtype=np.float32 if args.fp32c else np.float64)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -389,3 +389,19 @@ def test_disabled_direction_rejected(art_a):
         (grg,) = rt.grgs
         with pytest.raises(ValueError, match="DOWN plan is not configured"):
             grg.prepare_matmul_cuda(direction="down", k=1)
+
+
+def test_chromosome_batch_captured_in_one_graph(artifact_dir):
+    """Several GRGs with mixed k in one runtime: one captured graph for every op,
+    replays identical to the eager run, smallest GRGs against the oracle."""
+    import torch  # noqa: F401
+
+    from pygrgl_spmv.chromosomes import mixed_k, run
+    from pygrgl_spmv.synth import chromosome_workloads
+
+    cfgs = chromosome_workloads(5, 400, seed=77)
+    cfgs = [c.__class__(**{**c.__dict__, "num_mutations": max(500, c.num_mutations // 200), "num_trees": 30}) for c in cfgs]
+    arts = [write_synthetic_artifact(artifact_dir, f"chrom-{i}", c) for i, c in enumerate(cfgs)]
+    res = run(arts, budget_bytes=2_000_000_000, ring=4, replays=3)
+    assert res["captured"] and res["graph_replays_checked"] == 3
+    assert res["ks"] == [mixed_k(i) for i in range(5)]
