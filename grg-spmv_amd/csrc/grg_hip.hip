@@ -439,7 +439,8 @@ int validate(const GrgHipGatherArgs& a) {
   if (a.dtype != GRG_HIP_F64 && a.dtype != GRG_HIP_F32C) return fail("unknown dtype code " + std::to_string(a.dtype));
   if (a.row_end < a.row_begin) return fail("row_end < row_begin");
   if (a.k < 1) return fail("k must be >= 1");
-  if (a.cpl != 1 && a.cpl != 2 && a.cpl != 4) return fail("cpl must be 1, 2 or 4");
+  if (a.cpl != 1 && a.cpl != 2 && a.cpl != 4 && !(a.cpl == 8 && a.dtype == GRG_HIP_F32C))
+    return fail("cpl must be 1, 2 or 4 (or 8 for FP32 values: 32 bytes per lane)");
   if (!is_pow2(a.kl) || !is_pow2(a.es) || a.kl * a.es > 64) return fail("kl and es must be powers of two with kl * es <= 64");
   if (a.kl * a.cpl > 64) return fail("kl * cpl must be <= 64");
   {
@@ -519,6 +520,7 @@ int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
   } else {
     switch (a.cpl) {
       case 1: launch_unrolled<float, 1>(a, grid, n_light, g_shift, kl_shift, stream); break;
+      case 8: launch_unrolled<float, 8>(a, grid, n_light, g_shift, kl_shift, stream); break;
       case 2: launch_unrolled<float, 2>(a, grid, n_light, g_shift, kl_shift, stream); break;
       default: launch_unrolled<float, 4>(a, grid, n_light, g_shift, kl_shift, stream); break;
     }

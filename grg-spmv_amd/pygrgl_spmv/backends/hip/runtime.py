@@ -75,7 +75,7 @@ def column_shapes(dtype_code: int, k: int, lds) -> list[tuple[int, int]]:
     divide k and every leading dimension, kl lanes covering k (<= 64 columns per chunk)."""
     elem = 8 if dtype_code == ffi.F64 else 4
     out = []
-    for cpl in (4, 2, 1):
+    for cpl in (8, 4, 2, 1):
         if cpl * elem > 32 or k % cpl or any(int(ld) % cpl for ld in lds):
             continue
         out.append((cpl, min(_pow2_ceil(-(-k // cpl)), 64 // cpl)))
@@ -300,7 +300,7 @@ class HipRuntime:
         ``split_chunk``: rows of gather degree above ``split_min`` are summed in
         chunks of ``split_chunk`` entries by separate workgroups (the layout's
         scratch bound assumes the defaults). ``fixed_shape``: launch fields
-        (``unroll``, ``rows_per_group``, ``es``, ``flags``) forced on every launch
+        (``unroll``, ``rows_per_group``, ``es``, ``flags``, ``cpl``, ``kl``) forced on every launch
         instead of tuning, for tests of each kernel variant."""
         if int(split_min) < 1 or int(split_chunk) < 1:
             raise ValueError("split_min and split_chunk must be >= 1")
@@ -312,7 +312,7 @@ class HipRuntime:
         self.device = torch.device("cuda", int(layout.device))
         self.stream_ptr = int(layout.stream_ptr)
         self.autotune = bool(autotune) and not fixed_shape
-        allowed = {"unroll", "rows_per_group", "es", "flags"}
+        allowed = {"unroll", "rows_per_group", "es", "flags", "cpl", "kl"}
         if fixed_shape and set(fixed_shape) - allowed:
             raise ValueError(f"fixed_shape keys must be among {sorted(allowed)}")
         self.fixed_shape = dict(fixed_shape or {})

@@ -80,7 +80,7 @@ typedef struct GrgHipGatherArgs {
   int64_t row_begin;    /* rows [row_begin, row_end) are produced */
   int64_t row_end;
   int32_t k;            /* columns */
-  int32_t cpl;          /* columns per lane: 1, 2 or 4; k, every ld and every base must be multiples */
+  int32_t cpl;          /* columns per lane: 1, 2 or 4 (8 for GRG_HIP_F32C); k, every ld and every base must be multiples */
   int32_t kl;           /* lanes across one column chunk (power of two, kl * cpl columns per chunk) */
   int32_t es;           /* lanes interleaving over one row's entries (power of two, kl * es <= 64) */
   int32_t unroll;       /* gathers in flight per lane, row and round: 4, 8 or 16 (0 means 8) */

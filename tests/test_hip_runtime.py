@@ -137,6 +137,23 @@ def test_every_kernel_variant(art_a, oracle_a, unroll, rows, es):
         _check(grg.matmul(x, "up", by_individual=True, init="xtx"), O.matmul(oracle_a, x, "up", by_individual=True, init="xtx"))
 
 
+@pytest.mark.parametrize("unroll,rows", [(4, 1), (8, 1), (4, 2)])
+def test_fp32_wide_lanes(art_a, oracle_a, unroll, rows):
+    """FP32 values with 8 columns (32 bytes) per lane, k = 16 over 2 lanes."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], dtype=np.float32, requirements=full_requirements(max_k_up=16, max_k_down=16))
+    shape = {"cpl": 8, "kl": 2, "es": 2, "unroll": unroll, "rows_per_group": rows}
+    with HipRuntime(layout, fixed_shape=shape) as rt:
+        (grg,) = rt.grgs
+        rng = np.random.default_rng(3370 + unroll + rows)
+        for direction in ("up", "down"):
+            x = rng.choice(np.array([-1.0, 1.0], dtype=np.float32), size=(16, _cols(oracle_a, direction)))
+            np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x.astype(np.float64), direction))
+            z = rng.standard_normal((16, _cols(oracle_a, direction))).astype(np.float32)
+            assert O.normwise_rel_err(grg.matmul(z, direction), O.matmul(oracle_a, z.astype(np.float64), direction)) <= 1e-5
+
+
 # ---------------------------------------------------------------------------
 # streamed blocks through the pinned ring
 # ---------------------------------------------------------------------------
