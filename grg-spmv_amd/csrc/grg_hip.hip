@@ -273,6 +273,46 @@ __device__ __forceinline__ void store_row(const GrgHipGatherArgs& a, int64_t row
   store_raw<T, CPL>(static_cast<T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
 }
 
+// One row by the whole workgroup (heavy rows; rows of direct tiles): 256 / kl
+// edge lanes over the row's segments (or, for a split heavy row h, over its
+// FP64 partial rows), a shuffle + LDS reduction, one store. Uses partial[] and
+// one barrier: callers looping over rows add a barrier between rows.
+template <typename T, int CPL, int U, typename Acc>
+__device__ __forceinline__ void workgroup_row(const GrgHipGatherArgs& a, int64_t row, int64_t h, int kl_shift,
+                                              Acc (&partial)[kWavesPerBlock][64]) {
+  const int KL = 1 << kl_shift;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int cl = threadIdx.x & (KL - 1);
+  const int e = threadIdx.x >> kl_shift;
+  const int ES = kBlock >> kl_shift;
+  const int col = ((int)blockIdx.y * KL + cl) * CPL;
+  const bool live = col < a.k;
+  Acc acc[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) acc[c].zero();
+  if (live) {
+    const int32_t it0 = (h >= 0 && a.split_ptr) ? a.split_ptr[h] : 0;
+    const int32_t it1 = (h >= 0 && a.split_ptr) ? a.split_ptr[h + 1] : 0;
+    if (it1 > it0) {
+      // split row: sum the FP64 partial rows written by the partials launch
+      const double* part = static_cast<const double*>(a.partials) + col;
+      for (int32_t it = it0 + e; it < it1; it += ES) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[c].add_wide(part[(int64_t)it * a.k + c]);
+      }
+    } else {
+      for (int s = 0; s < a.nseg; ++s) {
+        const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
+        gather_entries<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, ptr[0] + e, ptr[1], ES, acc);
+      }
+    }
+    if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
+  }
+  block_reduce<CPL>(acc, KL, partial);
+  if (wave == 0 && lane < KL && live) store_row<T, CPL>(a, row, col, acc);
+}
+
 template <typename T, int CPL, int U, int R>
 __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherArgs a, int64_t n_light_blocks,
                                                              int g_shift, int kl_shift) {
@@ -395,32 +435,183 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
 
   // ---- heavy path: the whole workgroup on one row ----
   const int64_t h = (int64_t)blockIdx.x - n_light_blocks;
-  const int64_t row = a.heavy_rows[h];
-  const int cl = threadIdx.x & (KL - 1);
-  const int e = threadIdx.x >> kl_shift;
-  const int ES = kBlock >> kl_shift;
-  const int col = ((int)blockIdx.y * KL + cl) * CPL;
-  const bool live = col < a.k;
-  if (live) {
-    const int32_t it0 = a.split_ptr ? a.split_ptr[h] : 0;
-    const int32_t it1 = a.split_ptr ? a.split_ptr[h + 1] : 0;
-    if (it1 > it0) {
-      // split row: sum the FP64 partial rows written by the partials launch
-      const double* part = static_cast<const double*>(a.partials) + col;
-      for (int32_t it = it0 + e; it < it1; it += ES) {
+  workgroup_row<T, CPL, U>(a, a.heavy_rows[h], h, kl_shift, partial);
+}
+
+// Padding (elements) after each staged row of a tile: at least 16 bytes and one
+// lane vector, so rows start on a vector boundary and consecutive rows start in
+// different LDS banks.
+template <typename T, int CPL>
+constexpr int tile_pad() { return CPL * (int)sizeof(T) >= 16 ? CPL : 16 / (int)sizeof(T); }
+
+template <typename T>
+constexpr int tile_static_lds() { return (int)sizeof(typename SumOf<T>::type) * kWavesPerBlock * 64; }
+
+// LDS of a tiled workgroup, in bytes: the staged rows, then the tile's row
+// pointers, then its entry indices (each part a multiple of 16 bytes).
+struct TileLds {
+  int64_t staged, rowptr, entries;
+  __host__ __device__ int64_t total() const { return staged + rowptr + entries; }
+};
+
+__host__ __device__ inline int64_t round16(int64_t b) { return (b + 15) & ~int64_t(15); }
+
+__host__ __device__ inline TileLds tile_lds_layout(int64_t staged_rows, int64_t stride_bytes, int64_t max_rows,
+                                                   int64_t max_entries) {
+  TileLds t;
+  t.staged = round16(staged_rows * stride_bytes);
+  t.rowptr = round16((max_rows + 1) * 4);
+  t.entries = round16((max_entries + 16) * 2);
+  return t;
+}
+
+// Tiled launch: workgroup t copies into LDS, with coalesced loads, the
+// distinct source rows of tile t (one column chunk of kl * cpl columns each),
+// the tile's row pointers and its 16-bit entry indices; its lane groups then
+// sum the tile's rows from LDS alone and write them. Rows of one tile share
+// many sources, so each source row crosses L2 once per tile instead of once
+// per entry, and no global load sits between a row and its sum. Heavy rows
+// sit in direct tiles (skipped here) and follow as ordinary workgroup rows.
+template <typename T, int CPL>
+__global__ __launch_bounds__(kBlock) void tile_gather_kernel(const GrgHipGatherArgs a, int g_shift, int kl_shift) {
+  using Acc = typename SumOf<T>::type;
+  extern __shared__ __align__(16) unsigned char tile_lds[];
+  __shared__ Acc partial[kWavesPerBlock][64];
+  const int lane = threadIdx.x & 63;
+  const int KL = 1 << kl_shift;
+  const int64_t nt = a.num_tiles;
+  int64_t t = blockIdx.x;
+  if (t >= nt) {
+    workgroup_row<T, CPL, 8>(a, a.heavy_rows[t - nt], t - nt, kl_shift, partial);
+    return;
+  }
+  if (a.flags & GRG_HIP_FLAG_XCD_REMAP) {
+    const int64_t q = nt >> 3, r = nt & 7, x = t & 7;
+    t = x * q + (x < r ? x : r) + (t >> 3);
+  }
+  const int32_t r0 = a.tile_row[t], r1 = a.tile_row[t + 1];
+  const int32_t s0 = a.tile_src[t], s1 = a.tile_src[t + 1];
+  if (s1 == s0) {
+    // direct tile (its rows stage nothing): one workgroup per row, from the segments
+    for (int32_t r = r0; r < r1; ++r) {
+      const int32_t deg = a.eptr[r + 1] - a.eptr[r];
+      if (a.heavy_threshold >= 0 && deg > a.heavy_threshold) continue;
+      workgroup_row<T, CPL, 8>(a, r, -1, kl_shift, partial);
+      __syncthreads();  // partial[] is reused by the next row
+    }
+    return;
+  }
+  const int chunk = KL * CPL;
+  const int stride = chunk + tile_pad<T, CPL>();
+  const int col0 = (int)blockIdx.y * chunk;
+  const TileLds lay = tile_lds_layout(a.tile_lds_rows, (int64_t)stride * (int64_t)sizeof(T), a.tile_max_rows, a.tile_max_entries);
+  T* buf = reinterpret_cast<T*>(tile_lds);
+  int32_t* rowptr = reinterpret_cast<int32_t*>(tile_lds + lay.staged);
+  uint16_t* ent = reinterpret_cast<uint16_t*>(tile_lds + lay.staged + lay.rowptr);
+  const int32_t nrows = r1 - r0;
+  const int32_t e0 = a.eptr[r0];
+  const int32_t e1 = a.eptr[r1];
+  const int32_t eo = e0 & 7;  // entry e0 sits at ent[eo]
+  {
+    // ---- row pointers and entry indices (16-byte words of 8 indices) ----
+    for (int i = threadIdx.x; i <= nrows; i += kBlock) rowptr[i] = a.eptr[r0 + i] - e0;
+    typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+    const U4* src_w = reinterpret_cast<const U4*>(a.lidx) + (e0 >> 3);
+    U4* dst_w = reinterpret_cast<U4*>(ent);
+    const int n_words = ((e1 + 7) >> 3) - (e0 >> 3);
+    for (int i = threadIdx.x; i < n_words; i += kBlock) dst_w[i] = src_w[i];
+  }
+  {
+    // ---- staged source rows: D rows x KL lane vectors, 4 loads in flight per thread ----
+    const T* src0 = static_cast<const T*>(a.seg[0].src);
+    const T* src1 = a.nseg > 1 ? static_cast<const T*>(a.seg[1].src) : src0;
+    const T* src2 = a.nseg > 2 ? static_cast<const T*>(a.seg[2].src) : src0;
+    const int64_t ld0 = a.seg[0].ld;
+    const int64_t ld1 = a.nseg > 1 ? a.seg[1].ld : ld0;
+    const int64_t ld2 = a.nseg > 2 ? a.seg[2].ld : ld0;
+    const int n_pieces = (s1 - s0) << kl_shift;
+    for (int i = threadIdx.x; i < n_pieces; i += 4 * kBlock) {
+      T v[4][CPL];
+      int dst[4];
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) acc[c].add_wide(part[(int64_t)it * a.k + c]);
+      for (int u = 0; u < 4; ++u) {
+        const int ii = i + u * kBlock;
+        const int d = ii >> kl_shift, p = ii & (KL - 1);
+        const int col = col0 + p * CPL;
+        dst[u] = -1;
+        if (ii < n_pieces && col < a.k) {
+          const uint32_t key = a.src_list[s0 + d];
+          const uint32_t sg = key >> 30;
+          const int64_t srow = (int64_t)(key & 0x3fffffffu);
+          const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
+          const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
+          load_raw<T, CPL>(base + srow * ld + col, v[u]);
+          dst[u] = d * stride + p * CPL;
+        }
       }
-    } else {
-      for (int s = 0; s < a.nseg; ++s) {
-        const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
-        gather_entries<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, ptr[0] + e, ptr[1], ES, acc);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (dst[u] >= 0) store_raw<T, CPL>(buf + dst[u], v[u]);
+    }
+  }
+  __syncthreads();
+  // ---- sum: lane groups of G = kl * es lanes, one row each per pass, from LDS ----
+  const int G = 1 << g_shift;
+  const int lg = lane & (G - 1);
+  const int cl = lg & (KL - 1);
+  const int e = lg >> kl_shift;
+  const int ES = G >> kl_shift;
+  const int groups = kBlock >> g_shift;
+  const int gid = (int)threadIdx.x >> g_shift;
+  const int col = col0 + cl * CPL;
+  const T* lds_col = buf + cl * CPL;
+  const uint16_t* ent0 = ent + eo;
+  for (int32_t b = 0; b < nrows; b += groups) {  // same trip count for every lane (shuffles below)
+    const int32_t i = b + gid;
+    bool live = i < nrows && col < a.k;
+    Acc acc[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) acc[c].zero();
+    int32_t p0 = 0, p1 = 0;
+    if (live) {
+      p0 = rowptr[i];
+      p1 = rowptr[i + 1];
+      if (a.heavy_threshold >= 0 && p1 - p0 > a.heavy_threshold) live = false;
+    }
+    if (live) {
+      for (int32_t j = p0 + e; j < p1; j += 4 * ES) {
+        int li[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int32_t jj = j + u * ES;
+          li[u] = jj < p1 ? (int)ent0[jj] : -1;
+        }
+        T v[4][CPL];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (li[u] >= 0) {
+            load_raw<T, CPL>(lds_col + li[u] * stride, v[u]);
+          } else {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) v[u][c] = T(0);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) acc[c].add(v[u][c]);
       }
     }
-    if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
+    for (int off = KL; off < G; off <<= 1) {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) acc[c].merge_xor(off);
+    }
+    if (live && e == 0) {
+      const int32_t r = r0 + i;
+      add_row_terms<T, CPL>(a, r, col, acc);
+      store_row<T, CPL>(a, r, col, acc);
+    }
   }
-  block_reduce<CPL>(acc, KL, partial);
-  if (wave == 0 && lane < KL && live) store_row<T, CPL>(a, row, col, acc);
 }
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
@@ -433,6 +624,18 @@ int ilog2(int v) {
 bool misaligned(const void* p, int64_t ld, int cpl, size_t elem) {
   if (!p) return false;
   return (reinterpret_cast<uintptr_t>(p) % (cpl * elem)) != 0 || (ld % cpl) != 0;
+}
+
+constexpr int64_t kTileLdsLimit = 160 * 1024;  // LDS per CU on gfx950, including the kernel's static LDS
+
+// Dynamic LDS of a tiled launch plus the kernel's static LDS.
+int64_t tile_lds_bytes(const GrgHipGatherArgs& a) {
+  const bool f64 = a.dtype == GRG_HIP_F64;
+  const int64_t elem = f64 ? 8 : 4;
+  const int64_t pad = a.cpl * elem >= 16 ? a.cpl : 16 / elem;
+  const int64_t stride = (int64_t)a.kl * a.cpl + pad;
+  const int64_t stat = f64 ? tile_static_lds<double>() : tile_static_lds<float>();
+  return tile_lds_layout(a.tile_lds_rows, stride * elem, a.tile_max_rows, a.tile_max_entries).total() + stat;
 }
 
 int validate(const GrgHipGatherArgs& a) {
@@ -449,7 +652,8 @@ int validate(const GrgHipGatherArgs& a) {
     const bool ok = (r == 1 && (u == 4 || u == 8 || u == 16)) || (r == 2 && (u == 4 || u == 8)) || (r == 4 && u == 4);
     if (!ok) return fail("(unroll, rows_per_group) must be one of (4|8|16, 1), (4|8, 2), (4, 4)");
   }
-  if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS)) return fail("unknown flags");
+  if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_TILED)) return fail("unknown flags");
+  if ((a.flags & GRG_HIP_FLAG_TILED) && (a.flags & GRG_HIP_FLAG_PARTIALS)) return fail("a launch is either tiled or a partials launch");
   if (a.k % a.cpl) return fail("k must be a multiple of cpl");
   if (!a.out) return fail("out is null");
   if (a.nseg < 0 || a.nseg > GRG_HIP_MAX_SEGS) return fail("nseg out of range");
@@ -474,6 +678,16 @@ int validate(const GrgHipGatherArgs& a) {
     if (a.split_chunk < 1 || a.split_chunk > 0x40000000LL) return fail("split_chunk out of range");
     if (reinterpret_cast<uintptr_t>(a.partials) % (a.cpl * 8)) return fail("partials not aligned to cpl doubles");
   }
+  if (a.flags & GRG_HIP_FLAG_TILED) {
+    if (!a.tile_row || !a.tile_src || !a.src_list || !a.eptr || !a.lidx) return fail("tiled launch needs tile_row, tile_src, src_list, eptr and lidx");
+    if (a.nseg < 1) return fail("tiled launch needs at least one segment");
+    if (a.num_tiles < 0) return fail("num_tiles < 0");
+    if (a.tile_lds_rows < 0 || a.tile_lds_rows > GRG_HIP_TILE_MAX_STAGED) return fail("tile_lds_rows out of range");
+    if (a.tile_max_rows < 0 || a.tile_max_entries < 0) return fail("tile_max_rows / tile_max_entries < 0");
+    if (reinterpret_cast<uintptr_t>(a.lidx) % 16) return fail("lidx must be 16-byte aligned");
+    if ((int64_t)a.kl * a.cpl * (int64_t)elem > GRG_HIP_TILE_CHUNK_BYTES) return fail("tiled launch: kl * cpl columns exceed the staged row bytes");
+    if (tile_lds_bytes(a) > kTileLdsLimit) return fail("tiled launch: the staged rows exceed the LDS");
+  }
   return 0;
 }
 
@@ -495,11 +709,50 @@ void launch_unrolled(const GrgHipGatherArgs& a, dim3 grid, int64_t n_light, int 
   else launch_typed<T, CPL, 8, 1>(a, grid, n_light, g_shift, kl_shift, s);
 }
 
+template <typename T, int CPL>
+int launch_tiled_typed(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl_shift, hipStream_t s) {
+  const int64_t stat = tile_static_lds<T>();
+  const size_t dyn = (size_t)(tile_lds_bytes(a) - stat);
+  if (dyn > 64 * 1024) {
+    hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&tile_gather_kernel<T, CPL>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (err != hipSuccess) return fail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(err), 2);
+  }
+  hipLaunchKernelGGL((tile_gather_kernel<T, CPL>), grid, dim3(kBlock), dyn, s, a, g_shift, kl_shift);
+  return 0;
+}
+
 int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
   if (int rc = validate(a)) return rc;
   const int G = a.kl * a.es;
   const int g_shift = ilog2(G), kl_shift = ilog2(a.kl);
   int64_t n_light = 0, gx = 0;
+  if (a.flags & GRG_HIP_FLAG_TILED) {
+    gx = a.num_tiles + a.num_heavy;
+    if (gx == 0) return 0;
+    if (gx > 0x7fffffffLL) return fail("grid too large");
+    const int chunk = a.kl * a.cpl;
+    dim3 grid((unsigned)gx, (unsigned)((a.k + chunk - 1) / chunk));
+    int rc = 0;
+    if (a.dtype == GRG_HIP_F64) {
+      switch (a.cpl) {
+        case 1: rc = launch_tiled_typed<double, 1>(a, grid, g_shift, kl_shift, stream); break;
+        case 2: rc = launch_tiled_typed<double, 2>(a, grid, g_shift, kl_shift, stream); break;
+        default: rc = launch_tiled_typed<double, 4>(a, grid, g_shift, kl_shift, stream); break;
+      }
+    } else {
+      switch (a.cpl) {
+        case 1: rc = launch_tiled_typed<float, 1>(a, grid, g_shift, kl_shift, stream); break;
+        case 2: rc = launch_tiled_typed<float, 2>(a, grid, g_shift, kl_shift, stream); break;
+        case 8: rc = launch_tiled_typed<float, 8>(a, grid, g_shift, kl_shift, stream); break;
+        default: rc = launch_tiled_typed<float, 4>(a, grid, g_shift, kl_shift, stream); break;
+      }
+    }
+    if (rc) return rc;
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return fail(std::string("kernel launch failed: ") + hipGetErrorString(err), 2);
+    return 0;
+  }
   if (a.flags & GRG_HIP_FLAG_PARTIALS) {
     gx = a.split_items;
   } else {

@@ -2,6 +2,7 @@
 // order inside each level (see include/grg_host.h).
 
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <cstring>
 #include <numeric>
@@ -97,6 +98,97 @@ void locality_order(int64_t N, int64_t S, int32_t L, const int64_t* off, const G
   }
 }
 
+void build_tiles(int64_t R, int32_t L, const int64_t* off, int32_t nseg, const int32_t* const* sp,
+                 const int32_t* const* si, const int64_t* src_rows, const int32_t* eptr, const int64_t* heavy,
+                 int32_t max_staged, int32_t max_rows, int32_t max_entries, int32_t* tile_row, int32_t* tile_src,
+                 uint32_t* src_list, uint16_t* lidx, int64_t* level_tile_off, int32_t* level_max) {
+  if (R < 0 || L < 0 || nseg < 1 || nseg > 3) throw std::invalid_argument("bad sizes");
+  if (max_staged < 1 || max_staged > 65535 || max_rows < 1 || max_entries < 1) throw std::invalid_argument("bad tile limits");
+  if (off[0] != 0 || off[L] != R) throw std::invalid_argument("level offsets do not cover the rows");
+  // stamp[s][src] = id of the last tile that staged src, pos[s][src] = its staged position there
+  std::vector<std::vector<int32_t>> stamp(nseg);
+  std::vector<std::vector<uint16_t>> pos(nseg);
+  for (int32_t s = 0; s < nseg; ++s) {
+    if (src_rows[s] < 0 || src_rows[s] >= (int64_t(1) << 30)) throw std::invalid_argument("source rows must be below 2^30");
+    stamp[s].assign((size_t)src_rows[s], -1);
+    pos[s].assign((size_t)src_rows[s], 0);
+  }
+  int64_t nt = 0;  // tiles so far (the open tile, if any, is tile nt)
+  int64_t ns = 0;  // staged sources so far
+  int64_t cur_rows = 0, cur_entries = 0;
+  int32_t cur_staged = 0;
+  bool open = false;
+  int32_t* lvl_staged = level_max;
+  int32_t* lvl_rows = level_max + L;
+  int32_t* lvl_entries = level_max + 2 * L;
+  int32_t lvl = 0;
+  auto close = [&]() {
+    if (!open) return;
+    if (cur_staged > 0) {  // staged tile: record the level's LDS needs
+      lvl_staged[lvl] = std::max<int32_t>(lvl_staged[lvl], cur_staged);
+      lvl_rows[lvl] = std::max<int32_t>(lvl_rows[lvl], (int32_t)cur_rows);
+      lvl_entries[lvl] = std::max<int32_t>(lvl_entries[lvl], (int32_t)cur_entries);
+    }
+    ++nt;
+    open = false;
+  };
+  auto start = [&](int64_t r) {
+    tile_row[nt] = (int32_t)r;
+    tile_src[nt] = (int32_t)ns;
+    cur_rows = cur_entries = 0;
+    cur_staged = 0;
+    open = true;
+  };
+  for (int32_t l = 0; l < L; ++l) {
+    lvl = l;
+    level_tile_off[l] = nt;
+    lvl_staged[l] = lvl_rows[l] = lvl_entries[l] = 0;
+    const int64_t lo = off[l], hi = off[l + 1];
+    for (int64_t r = lo; r < hi; ++r) {
+      const int64_t deg = (int64_t)eptr[r + 1] - eptr[r];
+      int64_t check = 0;
+      for (int32_t s = 0; s < nseg; ++s) check += (int64_t)sp[s][r + 1] - sp[s][r];
+      if (check != deg) throw std::invalid_argument("eptr does not match the segment degrees");
+      const bool is_heavy = heavy[l] >= 0 && deg > heavy[l];
+      if (is_heavy || deg > max_staged || deg > max_entries) {
+        // a direct tile of its own (a heavy row in it is produced by its heavy workgroup)
+        close();
+        start(r);
+        cur_rows = 1;
+        close();
+        continue;
+      }
+      if (open) {
+        int64_t fresh = 0;
+        for (int32_t s = 0; s < nseg; ++s)
+          for (int32_t j = sp[s][r]; j < sp[s][r + 1]; ++j) fresh += stamp[s][(size_t)si[s][j]] != (int32_t)nt;
+        if (cur_staged + fresh > max_staged || cur_rows + 1 > max_rows || cur_entries + deg > max_entries) close();
+      }
+      if (!open) start(r);
+      int64_t e = eptr[r];
+      for (int32_t s = 0; s < nseg; ++s) {
+        for (int32_t j = sp[s][r]; j < sp[s][r + 1]; ++j, ++e) {
+          const int32_t c = si[s][j];
+          if (c < 0 || c >= src_rows[s]) throw std::invalid_argument("segment source out of range");
+          if (stamp[s][(size_t)c] != (int32_t)nt) {
+            stamp[s][(size_t)c] = (int32_t)nt;
+            pos[s][(size_t)c] = (uint16_t)cur_staged++;
+            src_list[ns++] = ((uint32_t)s << 30) | (uint32_t)c;
+          }
+          lidx[e] = pos[s][(size_t)c];
+        }
+      }
+      ++cur_rows;
+      cur_entries += deg;
+    }
+    close();
+  }
+  level_tile_off[L] = nt;
+  tile_row[nt] = (int32_t)R;
+  tile_src[nt] = (int32_t)ns;
+  if (ns > INT32_MAX) throw std::invalid_argument("more than 2^31 - 1 staged sources");
+}
+
 }  // namespace
 
 extern "C" {
@@ -109,6 +201,26 @@ int grg_host_locality_order(int64_t num_nodes, int64_t num_samples, int32_t num_
   }
   try {
     locality_order(num_nodes, num_samples, num_levels, level_offsets, blocks, num_blocks, new_pos);
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 2;
+  }
+}
+
+int grg_host_build_tiles(int64_t num_rows, int32_t num_levels, const int64_t* level_offsets, int32_t nseg,
+                         const int32_t* const* seg_ptr, const int32_t* const* seg_idx, const int64_t* seg_src_rows,
+                         const int32_t* eptr, const int64_t* heavy_threshold, int32_t max_staged, int32_t max_rows,
+                         int32_t max_entries, int32_t* tile_row, int32_t* tile_src, uint32_t* src_list, uint16_t* lidx,
+                         int64_t* level_tile_off, int32_t* level_max) {
+  if (!level_offsets || !seg_ptr || !seg_idx || !seg_src_rows || !eptr || !heavy_threshold || !tile_row || !tile_src ||
+      !src_list || !lidx || !level_tile_off || !level_max) {
+    g_last_error = "null argument";
+    return 1;
+  }
+  try {
+    build_tiles(num_rows, num_levels, level_offsets, nseg, seg_ptr, seg_idx, seg_src_rows, eptr, heavy_threshold,
+                max_staged, max_rows, max_entries, tile_row, tile_src, src_list, lidx, level_tile_off, level_max);
     return 0;
   } catch (const std::exception& e) {
     g_last_error = e.what();

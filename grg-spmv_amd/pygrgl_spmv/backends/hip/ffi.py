@@ -13,11 +13,14 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 6
+ABI_VERSION = 7
 F64 = 0
 F32C = 1
 FLAG_XCD_REMAP = 2
 FLAG_PARTIALS = 4
+FLAG_TILED = 8
+TILE_CHUNK_BYTES = 128
+TILE_MAX_STAGED = 1024
 MAX_SEGS = 3
 MAX_MAPS = 2
 
@@ -63,6 +66,15 @@ class GrgHipGatherArgs(ctypes.Structure):
         ("partials", c_void_p),
         ("out_scale", c_void_p),
         ("out_scale_base", c_int64),
+        ("tile_row", c_void_p),
+        ("tile_src", c_void_p),
+        ("src_list", c_void_p),
+        ("eptr", c_void_p),
+        ("lidx", c_void_p),
+        ("num_tiles", c_int64),
+        ("tile_lds_rows", c_int64),
+        ("tile_max_rows", c_int64),
+        ("tile_max_entries", c_int64),
     ]
 
 

@@ -3,6 +3,7 @@
 from __future__ import annotations
 
 import ctypes
+from dataclasses import dataclass
 from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
@@ -26,6 +27,10 @@ def _lib():
         lib = ctypes.CDLL(str(LIB_PATH))
         lib.grg_host_locality_order.argtypes = [c_int64, c_int64, c_int32, c_void_p, ctypes.POINTER(GrgHostBlock), c_int32, c_void_p]
         lib.grg_host_locality_order.restype = ctypes.c_int
+        lib.grg_host_build_tiles.argtypes = [c_int64, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                             c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                             c_void_p, c_void_p]
+        lib.grg_host_build_tiles.restype = ctypes.c_int
         lib.grg_host_last_error.argtypes = []
         lib.grg_host_last_error.restype = ctypes.c_char_p
         _LIB = lib
@@ -50,4 +55,81 @@ def locality_order(num_nodes: int, num_samples: int, level_offsets: np.ndarray, 
     return out
 
 
-__all__ = ["LIB_PATH", "locality_order"]
+@dataclass
+class Tiles:
+    """Tiles of one sweep configuration (``grg_host_build_tiles``): int32 ``tile_row`` /
+    ``tile_src`` (num_tiles + 1), uint32 ``src_list``, uint16 ``lidx`` (per merged
+    entry, plus LIDX_PAD zeros so 16-byte reads past the end stay inside), int32
+    ``eptr`` (rows + 1), per-level ``level_tile_off`` (L + 1), and per level the most
+    staged sources, rows and entries of any staged tile (``level_max_staged``,
+    ``level_max_rows``, ``level_max_entries``)."""
+
+    tile_row: np.ndarray
+    tile_src: np.ndarray
+    src_list: np.ndarray
+    lidx: np.ndarray
+    eptr: np.ndarray
+    level_tile_off: np.ndarray
+    level_max_staged: np.ndarray
+    level_max_rows: np.ndarray
+    level_max_entries: np.ndarray
+
+    @property
+    def nbytes(self) -> int:
+        return int(sum(a.nbytes for a in (self.tile_row, self.tile_src, self.src_list, self.lidx, self.eptr)))
+
+
+LIDX_PAD = 8
+
+
+def build_tiles(level_offsets: np.ndarray, segments, heavy_threshold, *, max_staged: int = 256, max_rows: int = 512,
+                max_entries: int = 4096) -> Tiles:
+    """Tiles over every level for segments given as (ptr int32[rows + 1], idx int32, source_rows)."""
+    lib = _lib()
+    off = np.ascontiguousarray(level_offsets, dtype=np.int64)
+    L = off.size - 1
+    R = int(off[-1])
+    nseg = len(segments)
+    if not 1 <= nseg <= 3:
+        raise ValueError("1 to 3 segments")
+    ptrs = [np.ascontiguousarray(p, dtype=np.int32) for p, _, _ in segments]
+    idxs = [np.ascontiguousarray(i, dtype=np.int32) for _, i, _ in segments]
+    for p in ptrs:
+        if p.size != R + 1:
+            raise ValueError("segment pointers must cover every row")
+    deg = np.zeros(R, dtype=np.int64)
+    for p in ptrs:
+        deg += np.diff(p.astype(np.int64))
+    eptr64 = np.zeros(R + 1, dtype=np.int64)
+    np.cumsum(deg, out=eptr64[1:])
+    total = int(eptr64[-1])
+    if total > np.iinfo(np.int32).max:
+        raise NotImplementedError("more than 2^31 - 1 entries in one sweep configuration")
+    eptr = eptr64.astype(np.int32)
+    src_rows = np.asarray([int(n) for _, _, n in segments], dtype=np.int64)
+    heavy = np.ascontiguousarray(heavy_threshold, dtype=np.int64)
+    if heavy.size != L:
+        raise ValueError("one heavy threshold per level")
+    cap_tiles = R + L + 1
+    tile_row = np.empty(cap_tiles, dtype=np.int32)
+    tile_src = np.empty(cap_tiles, dtype=np.int32)
+    src_list = np.empty(max(total, 1), dtype=np.uint32)
+    lidx = np.zeros(total + LIDX_PAD, dtype=np.uint16)
+    level_tile_off = np.empty(L + 1, dtype=np.int64)
+    level_max = np.zeros(max(3 * L, 1), dtype=np.int32)
+    pp = (c_void_p * nseg)(*[p.ctypes.data for p in ptrs])
+    ip = (c_void_p * nseg)(*[i.ctypes.data for i in idxs])
+    rc = lib.grg_host_build_tiles(R, L, off.ctypes.data, nseg, ctypes.cast(pp, c_void_p), ctypes.cast(ip, c_void_p),
+                                  src_rows.ctypes.data, eptr.ctypes.data, heavy.ctypes.data, int(max_staged), int(max_rows),
+                                  int(max_entries), tile_row.ctypes.data, tile_src.ctypes.data, src_list.ctypes.data, lidx.ctypes.data,
+                                  level_tile_off.ctypes.data, level_max.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"grg_host_build_tiles failed: status {rc}: {lib.grg_host_last_error().decode()}")
+    nt = int(level_tile_off[-1])
+    ns = int(tile_src[nt])
+    return Tiles(tile_row=tile_row[: nt + 1].copy(), tile_src=tile_src[: nt + 1].copy(), src_list=src_list[: max(ns, 1)].copy(),
+                 lidx=lidx, eptr=eptr, level_tile_off=level_tile_off, level_max_staged=level_max[:L].copy(),
+                 level_max_rows=level_max[L:2 * L].copy(), level_max_entries=level_max[2 * L:3 * L].copy())
+
+
+__all__ = ["LIB_PATH", "LIDX_PAD", "Tiles", "build_tiles", "locality_order"]

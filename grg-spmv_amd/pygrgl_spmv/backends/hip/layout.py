@@ -17,7 +17,8 @@ things: both sweep orientations are stored (no shared storage), selectors are
 int32 CSR (forward for the up write-out, transposed for the down seed), the
 resident blocks of each destination level are merged into one CSR whose row
 pointers are counted under ``metadata``, and the per-level row-batch tables
-and heavy-row lists get their own bounded category (``schedule``). ``bytes_total`` is an upper bound of what ``HipRuntime``
+and heavy-row lists get their own bounded category (``schedule``), as do the
+tile tables of the LDS-staged level launches (``tiles``). ``bytes_total`` is an upper bound of what ``HipRuntime``
 allocates for the layout.
 """
 
@@ -116,6 +117,7 @@ class HipLayout:
     bytes_total: int = 0
     heavy_threshold: int = HEAVY_THRESHOLD
     schedule_capacity: tuple[int, ...] = ()
+    tiles: bool = False
 
 
 def _resolve_artifacts(artifacts) -> tuple[Path, ...]:
@@ -250,6 +252,26 @@ def schedule_capacity_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeReq
     return int(configs * per_config)
 
 
+def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequirements) -> int:
+    """Upper bound of the tile tables of the tiled level launches.
+
+    Per sweep configuration (up; down; down with missing-data input): a uint16
+    staged index per gather entry (plus 16 bytes of padding), at most one uint32
+    staged source per entry,
+    int32 merged row pointers, and int32 tile row and staged-source offsets (at
+    most one tile per row plus one per level).
+    """
+    N, L, E = scan.num_nodes, scan.num_levels, scan.num_edges
+    entries = []
+    if pair.plan_up is not None:
+        entries.append(E)
+    if pair.plan_down is not None:
+        entries.append(E + scan.selector_mut_nnz)
+        if req.need_down_miss_input:
+            entries.append(E + scan.selector_mut_nnz + scan.selector_miss_nnz)
+    return int(sum(6 * e + 16 + 4 * (N + 1) + 8 * (N + L + 1) for e in entries))
+
+
 def plan_hip_layout(
     *,
     artifacts,
@@ -261,8 +283,13 @@ def plan_hip_layout(
     allow_residency: bool = True,
     device,
     stream,
+    tiles: bool = True,
 ) -> HipLayout:
-    """Plan device memory for ``artifacts`` (no device allocation happens here)."""
+    """Plan device memory for ``artifacts`` (no device allocation happens here).
+
+    ``tiles`` (no reference counterpart) reserves the tile tables of the
+    LDS-staged level launches (category ``tiles``, an upper bound); without it
+    the runtime runs every level with the per-row gather path."""
     dtype = np.dtype(dtype)
     if dtype not in (np.dtype(np.float32), np.dtype(np.float64)):
         raise ValueError(f"HIP runtime supports only float32/float64, got {dtype}")
@@ -291,7 +318,7 @@ def plan_hip_layout(
     I_max = max(s.num_individuals for s in scans)
     sel_max = max(max(s.selector_mut_nnz, s.selector_miss_nnz) for s in scans)
 
-    selector_bytes = metadata_bytes = xtx_bytes = heavy_bytes = 0
+    selector_bytes = metadata_bytes = xtx_bytes = heavy_bytes = tile_bytes = 0
     heavy_caps = []
     items_max = 0
     for s in scans:
@@ -299,6 +326,8 @@ def plan_hip_layout(
         metadata_bytes += _metadata_bytes(s, pair, requirements, dtype)
         if requirements.need_init_xtx and s.has_individual_coals:
             xtx_bytes += s.num_nodes * dtype.itemsize
+        if tiles:
+            tile_bytes += tile_bytes_for(s, pair, requirements)
         cap = schedule_capacity_for(s, pair, requirements)
         heavy_caps.append(cap)
         heavy_bytes += 4 * cap
@@ -325,7 +354,7 @@ def plan_hip_layout(
             blocks_up=ups, blocks_down=downs,
         ))
 
-    fixed = int(selector_bytes + metadata_bytes + xtx_bytes + heavy_bytes + dense_bytes + state_bytes + scratch_bytes)
+    fixed = int(selector_bytes + metadata_bytes + xtx_bytes + heavy_bytes + tile_bytes + dense_bytes + state_bytes + scratch_bytes)
     full_resident = int(sum(b.nbytes for b in blocks))
     required_full = fixed + full_resident
     effective = required_full if budget == 0 else budget
@@ -377,12 +406,13 @@ def plan_hip_layout(
         "metadata": int(metadata_bytes),
         "xtx_bias": int(xtx_bytes),
         "schedule": int(heavy_bytes),
+        "tiles": int(tile_bytes),
         "dense_arena": int(dense_bytes),
         "state": int(state_bytes),
         "scratch": int(scratch_bytes),
     }
     items = [BudgetItem(kind="fixed", name=name, nbytes=int(by_cat[name]))
-             for name in ("selectors", "metadata", "xtx_bias", "schedule", "dense_arena", "state", "scratch")]
+             for name in ("selectors", "metadata", "xtx_bias", "schedule", "tiles", "dense_arena", "state", "scratch")]
     for ai, al in enumerate(planned):
         for b in (*al.blocks_up, *al.blocks_down):
             if b.resident:
@@ -399,8 +429,8 @@ def plan_hip_layout(
         max_up_ops_by_level=max_up_ops, max_down_ops_by_level=max_down_ops, slot_plans=slots,
         budget_items=tuple(i for i in items if i.nbytes > 0), required_budget_for_full_residency=int(required_full),
         bytes_by_category=by_cat, bytes_total=int(sum(i.nbytes for i in items)),
-        heavy_threshold=HEAVY_THRESHOLD, schedule_capacity=tuple(heavy_caps),
+        heavy_threshold=HEAVY_THRESHOLD, schedule_capacity=tuple(heavy_caps), tiles=bool(tiles),
     )
 
 
-__all__ = ["HEAVY_THRESHOLD", "SPLIT_CHUNK", "SPLIT_MIN", "split_items_bound", "HipArtifactLayout", "HipBlockPlan", "HipLayout", "HipSlotPlan", "enabled_max_k", "plan_hip_layout", "side_io_rows"]
+__all__ = ["HEAVY_THRESHOLD", "SPLIT_CHUNK", "SPLIT_MIN", "split_items_bound", "HipArtifactLayout", "HipBlockPlan", "HipLayout", "HipSlotPlan", "enabled_max_k", "plan_hip_layout", "side_io_rows", "tile_bytes_for"]

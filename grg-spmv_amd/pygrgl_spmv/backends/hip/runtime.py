@@ -68,6 +68,9 @@ def _pow2_floor(v: float) -> int:
 
 
 KERNEL_UNROLL = 8  # gathers in flight per lane and round in the kernel
+TILE_MAX_STAGED = 256   # staged source rows of one tile (LDS rows)
+TILE_MAX_ROWS = 512     # rows of one tile
+TILE_MAX_ENTRIES = 4096  # gather entries of one tile
 
 
 def column_shapes(dtype_code: int, k: int, lds) -> list[tuple[int, int]]:
@@ -90,6 +93,22 @@ def launch_shape(dtype_code: int, k: int, lds, mean_degree: float) -> tuple[int,
     cpl, kl = next(((c, l) for c, l in shapes if c * elem <= 16), shapes[-1])
     es = max(1, min(_pow2_floor(max(mean_degree, 1.0) / KERNEL_UNROLL), 64 // kl))
     return cpl, kl, es
+
+
+TILE_LDS_LIMIT = 160 * 1024  # LDS per CU on gfx950
+
+
+def tile_lds_bytes(dtype_code: int, cpl: int, kl: int, staged_rows: int, max_rows: int, max_entries: int) -> int:
+    """LDS of one tiled workgroup: staged rows of kl * cpl columns plus a padding of at
+    least 16 bytes and one lane vector, the tile's row pointers and entry indices (each
+    part rounded to 16 bytes), and the 2 KiB reduction buffer."""
+    elem = 8 if dtype_code == ffi.F64 else 4
+    pad = cpl if cpl * elem >= 16 else 16 // elem
+
+    def r16(b):
+        return (int(b) + 15) // 16 * 16
+
+    return r16(int(staged_rows) * (kl * cpl + pad) * elem) + r16((int(max_rows) + 1) * 4) + r16((int(max_entries) + 16) * 2) + 8 * 4 * 64
 
 
 @dataclass(frozen=True)
@@ -178,6 +197,7 @@ class _HipArtifact:
     up_events: list = field(default_factory=list)     # per level: list[(copy_done, compute_done)]
     down_events: list = field(default_factory=list)
     max_items: int = 0                                # largest split-item count of one level launch
+    tiles: dict = field(default_factory=dict)         # config -> per-level tile offsets and LDS maxima
 
 
 def _i32(values, label: str) -> np.ndarray:
@@ -294,14 +314,21 @@ class HipRuntime:
     """Runtime-owned HIP execution with resident or streamed sparse blocks."""
 
     def __init__(self, layout: HipLayout, *, autotune: bool = True, split_min: int = SPLIT_MIN,
-                 split_chunk: int = SPLIT_CHUNK, fixed_shape: dict | None = None) -> None:
+                 split_chunk: int = SPLIT_CHUNK, fixed_shape: dict | None = None, tiles: bool = True,
+                 tile_max_staged: int = TILE_MAX_STAGED, tile_max_rows: int = TILE_MAX_ROWS,
+                 tile_max_entries: int = TILE_MAX_ENTRIES) -> None:
         """``autotune``: time the admissible launch shapes of every level launch when an
         op is prepared (otherwise a fixed default shape). ``split_min`` /
         ``split_chunk``: rows of gather degree above ``split_min`` are summed in
         chunks of ``split_chunk`` entries by separate workgroups (the layout's
         scratch bound assumes the defaults). ``fixed_shape``: launch fields
         (``unroll``, ``rows_per_group``, ``es``, ``flags``, ``cpl``, ``kl``) forced on every launch
-        instead of tuning, for tests of each kernel variant."""
+        instead of tuning, for tests of each kernel variant (a ``flags`` value with
+        ``ffi.FLAG_TILED`` applies to the level launches that have tiles). ``tiles``:
+        build the tile tables of the LDS-staged level launches (when the layout
+        reserved them) and let the tuner choose them per level; ``tile_max_staged``
+        / ``tile_max_rows`` / ``tile_max_entries`` bound the staged sources, rows and
+        gather entries of one tile."""
         if int(split_min) < 1 or int(split_chunk) < 1:
             raise ValueError("split_min and split_chunk must be >= 1")
         torch = _torch()
@@ -318,6 +345,12 @@ class HipRuntime:
         self.fixed_shape = dict(fixed_shape or {})
         self.split_min = int(split_min)
         self.split_chunk = int(split_chunk)
+        if not 1 <= int(tile_max_staged) <= ffi.TILE_MAX_STAGED or int(tile_max_rows) < 1 or int(tile_max_entries) < 1:
+            raise ValueError(f"tile_max_staged must be in [1, {ffi.TILE_MAX_STAGED}], tile_max_rows and tile_max_entries >= 1")
+        self.use_tiles = bool(tiles)
+        self.tile_max_staged = int(tile_max_staged)
+        self.tile_max_rows = int(tile_max_rows)
+        self.tile_max_entries = int(tile_max_entries)
         self._dtype_code = ffi.F64 if np.dtype(layout.dtype) == np.float64 else ffi.F32C
         self._reset_owned_state()
 
@@ -341,6 +374,7 @@ class HipRuntime:
         self._artifacts = ()
         self._grgs = ()
         self._tune_cache = {}
+        self.tune_log = {}
         self._sweep_timer = None
         self._entered = False
         self._active_call = False
@@ -500,7 +534,7 @@ class HipRuntime:
         return ptr_all.to(torch.int32), idx, deg
 
     def _build_artifact(self, ai: int, al, st) -> _HipArtifact:
-        from pygrgl_spmv.backends.hip.host import locality_order
+        from pygrgl_spmv.backends.hip.host import build_tiles, locality_order
 
         torch = _torch()
         lay = self.layout
@@ -581,13 +615,16 @@ class HipRuntime:
                 if req.need_up_miss_output:
                     D["miss_ptr"], D["miss_idx"] = self._to_dev(_i32(miss_ptr, "sel_miss.indptr")), self._to_dev(_i32(miss_idx, "sel_miss.indices"))
             deg_mut_t = deg_miss_t = None
+            sel_t_host = miss_t_host = None
             if lay.pair.plan_down is not None:
                 tp, ti = _csr_transpose(sel_ptr, sel_idx, M, N)
-                D["selT_ptr"], D["selT_idx"] = self._to_dev(_i32(tp, "sel_mut^T.indptr")), self._to_dev(_i32(ti, "sel_mut^T.indices"))
+                sel_t_host = (_i32(tp, "sel_mut^T.indptr"), _i32(ti, "sel_mut^T.indices"), M)
+                D["selT_ptr"], D["selT_idx"] = self._to_dev(sel_t_host[0]), self._to_dev(sel_t_host[1])
                 deg_mut_t = np.diff(tp)
                 if req.need_down_miss_input:
                     tp, ti = _csr_transpose(miss_ptr, miss_idx, M, N)
-                    D["missT_ptr"], D["missT_idx"] = self._to_dev(_i32(tp, "sel_miss^T.indptr")), self._to_dev(_i32(ti, "sel_miss^T.indices"))
+                    miss_t_host = (_i32(tp, "sel_miss^T.indptr"), _i32(ti, "sel_miss^T.indices"), M)
+                    D["missT_ptr"], D["missT_idx"] = self._to_dev(miss_t_host[0]), self._to_dev(miss_t_host[1])
                     deg_miss_t = np.diff(tp)
             # node / sample maps through the new order
             node_perm = np.empty(N, dtype=np.int64)
@@ -647,6 +684,31 @@ class HipRuntime:
             if cursor > cap:
                 raise RuntimeError(f"internal error: {cursor} schedule entries exceed the planned capacity {cap}")
             D["sched"] = self._to_dev(np.concatenate(tables).astype(np.int32) if tables else np.zeros(1, np.int32))
+            if lay.tiles and self.use_tiles:
+                # tile tables of the LDS-staged level launches, one set per sweep configuration
+                seg_host = {}
+                if "up_ptr" in D:
+                    seg_host["up"] = [(D["up_ptr"].cpu().numpy(), D["up_idx"].cpu().numpy(), N)]
+                if "dn_ptr" in D:
+                    dn = (D["dn_ptr"].cpu().numpy(), D["dn_idx"].cpu().numpy(), N)
+                    seg_host["down"] = [dn, sel_t_host]
+                    if miss_t_host is not None:
+                        seg_host["down_miss"] = [dn, sel_t_host, miss_t_host]
+                for name, segs_h in seg_host.items():
+                    if name not in configs:
+                        continue
+                    heavy = [art.sched[(name, l)].heavy_threshold for l in range(L)]
+                    t = build_tiles(off, segs_h, heavy, max_staged=self.tile_max_staged, max_rows=self.tile_max_rows,
+                                    max_entries=self.tile_max_entries)
+                    D[f"{name}_tile_row"] = self._to_dev(t.tile_row)
+                    D[f"{name}_tile_src"] = self._to_dev(t.tile_src)
+                    D[f"{name}_src_list"] = self._to_dev(t.src_list.view(np.int32))
+                    D[f"{name}_eptr"] = self._to_dev(t.eptr)
+                    D[f"{name}_lidx"] = self._to_dev(t.lidx.view(np.int16))
+                    art.tiles[name] = (t.level_tile_off.copy(), t.level_max_staged.copy(), t.level_max_rows.copy(),
+                                       t.level_max_entries.copy())
+                    del t
+                del seg_host
             # streamed ops with slot-reuse links, and their events
             art.up_stream = relink_stream_dependencies(up_stream, direction=Direction.UP)
             art.down_stream = relink_stream_dependencies(dn_stream, direction=Direction.DOWN)
@@ -659,10 +721,26 @@ class HipRuntime:
     def _elem(self) -> int:
         return 8 if self._dtype_code == ffi.F64 else 4
 
+    def _tile_fields(self, art: _HipArtifact, cfg: str, level: int):
+        """Tile fields of one level launch of sweep configuration ``cfg``, or None."""
+        tl = art.tiles.get(cfg)
+        if tl is None:
+            return None
+        level_tile_off, level_max_staged, level_max_rows, level_max_entries = tl
+        D = art.dev
+        t0, t1 = int(level_tile_off[level]), int(level_tile_off[level + 1])
+        if t1 <= t0:
+            return None
+        return {"tile_row": D[f"{cfg}_tile_row"].data_ptr() + 4 * t0, "tile_src": D[f"{cfg}_tile_src"].data_ptr() + 4 * t0,
+                "src_list": D[f"{cfg}_src_list"].data_ptr(), "eptr": D[f"{cfg}_eptr"].data_ptr(),
+                "lidx": D[f"{cfg}_lidx"].data_ptr(), "num_tiles": t1 - t0, "tile_lds_rows": int(level_max_staged[level]),
+                "tile_max_rows": int(level_max_rows[level]), "tile_max_entries": int(level_max_entries[level])}
+
     def _args(self, *, rows, k, out, ld_out, segs=(), maps=(), row_scale=None, col_vec=None, accumulate=False,
               sched: _LevelSchedule | None = None, sched_base: int = 0, mean_degree: float = 1.0,
-              out_base=0) -> ffi.GrgHipGatherArgs:
-        """Launch descriptor for rows [rows); ``sched`` adds the level's heavy rows."""
+              out_base=0, tiles: dict | None = None) -> ffi.GrgHipGatherArgs:
+        """Launch descriptor for rows [rows); ``sched`` adds the level's heavy rows,
+        ``tiles`` (from ``_tile_fields``) the level's tile tables."""
         a = ffi.GrgHipGatherArgs()
         a.dtype = self._dtype_code
         a.accumulate = 1 if accumulate else 0
@@ -684,8 +762,17 @@ class HipRuntime:
         a.rows_per_group = 1
         lds = [ld_out] + [s[3] for s in segs] + [m[2] for m in maps]
         a.cpl, a.kl, a.es = launch_shape(self._dtype_code, int(k), lds, mean_degree)
+        if tiles:
+            for name, value in tiles.items():
+                setattr(a, name, int(value))
         for name, value in self.fixed_shape.items():
             setattr(a, name, int(value))
+        if a.flags & ffi.FLAG_TILED:
+            if not tiles:
+                a.flags &= ~ffi.FLAG_TILED
+            else:
+                while a.kl > 1 and a.kl * a.cpl * self._elem() > ffi.TILE_CHUNK_BYTES:
+                    a.kl //= 2
         if a.kl * a.es > 64:
             a.es = 64 // a.kl
         a.out, a.ld_out, a.out_base = int(out), int(ld_out), int(out_base)
@@ -746,8 +833,28 @@ class HipRuntime:
                     time_shape(cpl, kl, es, unroll, rows, 0)
                 _, cpl, kl, es, unroll, rows, _ = min(timings)
                 time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_XCD_REMAP)
+                if a.tile_row:
+                    # the LDS-staged form of the same launch: column chunks of at most
+                    # TILE_CHUNK_BYTES, every lane-group width
+                    n_untiled = len(timings)
+                    for cpl, kl in column_shapes(a.dtype, a.k, lds):
+                        while kl > 1 and kl * cpl * elem > ffi.TILE_CHUNK_BYTES:
+                            kl //= 2
+                        if kl * cpl * elem > ffi.TILE_CHUNK_BYTES or tile_lds_bytes(a.dtype, cpl, kl, a.tile_lds_rows, a.tile_max_rows, a.tile_max_entries) > TILE_LDS_LIMIT:
+                            continue
+                        for es in (1, 2, 4, 8, 16):
+                            if kl * es <= 64:
+                                time_shape(cpl, kl, es, 8, 1, ffi.FLAG_TILED)
+                    if len(timings) > n_untiled:
+                        _, cpl, kl, es, unroll, rows, _ = min(timings[n_untiled:])
+                        time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_TILED | ffi.FLAG_XCD_REMAP)
             best = min(timings)[1:]
             self._tune_cache[key] = best
+            tiled = [t for t in timings if t[6] & ffi.FLAG_TILED]
+            untiled = [t for t in timings if not t[6] & ffi.FLAG_TILED]
+            self.tune_log[key] = {"rows": int(a.row_end - a.row_begin), "best": min(timings),
+                                  "untiled_ms": min(untiled)[0] if untiled else None,
+                                  "tiled_ms": min(tiled)[0] if tiled else None}
         if best is not None:
             a.cpl, a.kl, a.es, a.unroll, a.rows_per_group, a.flags = best
 
@@ -811,7 +918,8 @@ class HipRuntime:
                     segs.append((D["missT_ptr"].data_ptr(), D["missT_idx"].data_ptr(), io1, max_k, 0))
             sched = art.sched[(cfg, l)] if segs else None
             level_args = self._args(rows=(lo, hi), k=k, out=V, ld_out=ldv, segs=segs, maps=maps,
-                                    row_scale=row_scale_init, col_vec=col_vec_init, sched=sched, sched_base=sched_base)
+                                    row_scale=row_scale_init, col_vec=col_vec_init, sched=sched, sched_base=sched_base,
+                                    tiles=self._tile_fields(art, cfg, l) if segs else None)
             if segs:
                 self._tune(level_args, (art.path, cfg, l, k))
             if sched is not None and sched.num_items:

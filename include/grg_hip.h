@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 6
+#define GRG_HIP_ABI_VERSION 7
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -55,6 +55,12 @@ enum GrgHipDtype {
  * segments of heavy row h = split_row[i] (j = i - split_ptr[h]) and writes
  * the FP64 row partials[i * k + c]. No row terms, nothing else written. */
 #define GRG_HIP_FLAG_PARTIALS 4
+/* Tiled launch: see the tile fields of GrgHipGatherArgs. */
+#define GRG_HIP_FLAG_TILED 8
+/* Bytes of one staged row (kl * cpl columns) in a tiled launch, at most. */
+#define GRG_HIP_TILE_CHUNK_BYTES 128
+/* Staged rows of one tile, at most (uint16 LDS indices). */
+#define GRG_HIP_TILE_MAX_STAGED 1024
 
 /* sum_{j in [ptr[row - ptr_base], ptr[row - ptr_base + 1])} src[idx[j] * ld + c] */
 typedef struct GrgHipSeg {
@@ -119,6 +125,30 @@ typedef struct GrgHipGatherArgs {
    * out_scale[row - out_scale_base] after all its terms are summed. */
   const void* out_scale;
   int64_t out_scale_base;
+  /* Tiled launch (GRG_HIP_FLAG_TILED; no reference counterpart: an LDS-staged
+   * form of the same row sums). Rows [row_begin, row_end) are cut into
+   * num_tiles tiles of consecutive rows, tile t = rows [tile_row[t],
+   * tile_row[t + 1]). Its distinct sources src_list[tile_src[t] ..
+   * tile_src[t + 1]) (each (segment << 30) | source row) are copied once into
+   * LDS together with its row pointers and entry indices, and row r sums the
+   * staged rows lidx[j] for j in [eptr[r], eptr[r + 1]) (the row's entries over
+   * all segments, in segment order; eptr is indexed by global row; lidx is
+   * 16-byte aligned and readable 8 entries past the last one). A tile with no
+   * staged sources sums its rows straight from the segments, one workgroup per
+   * row. Heavy rows (as above) are skipped by the tiles and produced by the
+   * heavy workgroups after them: grid = num_tiles + num_heavy. Over the staged
+   * tiles of the launch, tile_lds_rows bounds the staged rows, tile_max_rows the
+   * rows and tile_max_entries the entries of a tile (they size the LDS); kl * cpl
+   * columns of the value type must fit in 128 bytes. */
+  const int32_t* tile_row;
+  const int32_t* tile_src;
+  const uint32_t* src_list;
+  const int32_t* eptr;
+  const uint16_t* lidx;
+  int64_t num_tiles;
+  int64_t tile_lds_rows;
+  int64_t tile_max_rows;
+  int64_t tile_max_entries;
 } GrgHipGatherArgs;
 
 /* One row-gather launch (see above). Replaces one level of the reference's

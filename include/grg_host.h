@@ -33,6 +33,40 @@ typedef struct GrgHostBlock {
 int grg_host_locality_order(int64_t num_nodes, int64_t num_samples, int32_t num_levels, const int64_t* level_offsets,
                             const GrgHostBlock* blocks, int32_t num_blocks, int64_t* new_pos);
 
+/* Tiles of consecutive rows for the tiled launch of libgrg_hip.so (see the
+ * tile fields of GrgHipGatherArgs in grg_hip.h), for every level of one sweep
+ * configuration at once. No reference counterpart.
+ *
+ * Rows are the num_rows device-order nodes; level l is rows
+ * [level_offsets[l], level_offsets[l + 1]). Row r gathers the entries of
+ * nseg segments, segment s holding seg_idx[s][seg_ptr[s][r] .. seg_ptr[s][r + 1])
+ * (source rows in [0, seg_src_rows[s]), < 2^30); eptr[r] (num_rows + 1
+ * offsets, computed by the caller) is where row r's entries start in the
+ * concatenation over segments.
+ *
+ * Per level, rows are taken in order. A row of degree > heavy_threshold[l]
+ * (when >= 0; it is produced by a heavy workgroup), > max_staged or
+ * > max_entries gets a tile of its own with nothing staged ("direct" tile).
+ * Every other row joins the current tile unless that would raise the tile's
+ * distinct sources above max_staged, its rows above max_rows or its entries
+ * above max_entries. A tile never spans two levels; levels are tiled even
+ * when their rows have no entries.
+ *
+ * Outputs (capacities in brackets):
+ *   tile_row  [num_rows + num_levels + 1]: tile t = rows [tile_row[t], tile_row[t + 1]);
+ *   tile_src  [num_rows + num_levels + 1]: tile t stages src_list[tile_src[t] .. tile_src[t + 1]);
+ *   src_list  [eptr[num_rows]]: (segment << 30) | source row, first-use order within a tile;
+ *   lidx      [eptr[num_rows]]: staged position of each entry (written for rows of staged tiles);
+ *   level_tile_off [num_levels + 1]: level l's tiles are [level_tile_off[l], level_tile_off[l + 1]);
+ *   level_max [3 * num_levels]: over the staged tiles of level l, the most staged
+ *     sources (level_max[l]), rows (level_max[L + l]) and entries (level_max[2L + l]).
+ * Returns 0 on success. */
+int grg_host_build_tiles(int64_t num_rows, int32_t num_levels, const int64_t* level_offsets, int32_t nseg,
+                         const int32_t* const* seg_ptr, const int32_t* const* seg_idx, const int64_t* seg_src_rows,
+                         const int32_t* eptr, const int64_t* heavy_threshold, int32_t max_staged, int32_t max_rows,
+                         int32_t max_entries, int32_t* tile_row, int32_t* tile_src, uint32_t* src_list, uint16_t* lidx,
+                         int64_t* level_tile_off, int32_t* level_max);
+
 const char* grg_host_last_error(void);
 
 #ifdef __cplusplus

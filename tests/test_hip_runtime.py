@@ -154,6 +154,55 @@ def test_fp32_wide_lanes(art_a, oracle_a, unroll, rows):
             assert O.normwise_rel_err(grg.matmul(z, direction), O.matmul(oracle_a, z.astype(np.float64), direction)) <= 1e-5
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("es,remap", [(1, False), (4, True), (16, False)])
+@pytest.mark.parametrize("max_staged,max_rows", [(24, 16), (512, 1024)])
+def test_tiled_level_launches(art_a, oracle_a, dtype, es, remap, max_staged, max_rows):
+    """Every level launch in its LDS-staged tiled form (small tiles, direct tiles for rows
+    longer than max_staged, heavy and split rows beside the tiles), against the oracle in
+    every sweep configuration: exact for +-1 inputs, FP64 within 1e-12, FP32c within 1e-5."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+    from pygrgl_spmv.backends.hip import ffi
+
+    layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
+    assert layout.tiles and layout.bytes_by_category["tiles"] > 0
+    flags = ffi.FLAG_TILED | (ffi.FLAG_XCD_REMAP if remap else 0)
+    shape = {"es": es, "flags": flags}
+    tol = FP64_TOL if dtype == np.float64 else 1e-5
+    with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16, tile_max_staged=max_staged,
+                    tile_max_rows=max_rows) as rt:
+        (grg,) = rt.grgs
+        rng = np.random.default_rng(3390 + es + max_staged)
+        for k in (3, 16):
+            for direction in ("up", "down"):
+                x = rng.choice(np.array([-1.0, 1.0]), size=(k, _cols(oracle_a, direction))).astype(dtype)
+                np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x.astype(np.float64), direction))
+        y = rng.standard_normal((8, oracle_a.num_mutations))
+        miss = rng.standard_normal((8, oracle_a.num_mutations))
+        got = grg.matmul(y.astype(dtype), "down", miss=miss.astype(dtype))
+        assert O.normwise_rel_err(got, O.matmul(oracle_a, y, "down", miss=miss)) <= tol
+        x = rng.standard_normal((8, oracle_a.num_individuals))
+        got = grg.matmul(x.astype(dtype), "up", by_individual=True, init="xtx")
+        assert O.normwise_rel_err(got, O.matmul(oracle_a, x, "up", by_individual=True, init="xtx")) <= tol
+
+
+def test_tuner_may_choose_tiles(art_a, oracle_a):
+    """With tuning, each level picks the faster of its per-row and tiled forms; results
+    agree with the oracle either way, and without tile tables no launch is tiled."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=16, max_k_down=16))
+    rng = np.random.default_rng(3399)
+    x = rng.standard_normal((16, oracle_a.num_samples))
+    y = rng.standard_normal((16, oracle_a.num_mutations))
+    for tiles in (True, False):
+        with HipRuntime(layout, tiles=tiles) as rt:
+            (grg,) = rt.grgs
+            _check(grg.matmul(x, "up"), O.matmul(oracle_a, x, "up"))
+            _check(grg.matmul(y, "down"), O.matmul(oracle_a, y, "down"))
+            assert tiles or not rt._artifacts[0].tiles
+
+
 # ---------------------------------------------------------------------------
 # streamed blocks through the pinned ring
 # ---------------------------------------------------------------------------

@@ -1,0 +1,111 @@
+"""Host tile builder (``grg_host_build_tiles``): structure invariants, and the
+tiled row sums restated on the CPU equal the plain CSR row sums."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from pygrgl_spmv.backends.hip.host import build_tiles
+
+
+def _random_csr(rng, rows, src_rows, max_deg, heavy_rows=()):
+    deg = rng.integers(0, max_deg + 1, size=rows)
+    for r, d in heavy_rows:
+        deg[r] = d
+    deg = np.minimum(deg, src_rows)
+    ptr = np.zeros(rows + 1, dtype=np.int64)
+    np.cumsum(deg, out=ptr[1:])
+    idx = np.concatenate([np.sort(rng.choice(src_rows, size=int(d), replace=False)) for d in deg] or [np.zeros(0, int)])
+    return ptr.astype(np.int32), idx.astype(np.int32)
+
+
+def _tiled_sums(t, segs, values, off, heavy):
+    """Row sums through the tiles, as the tiled launch computes them (heavy rows left at 0)."""
+    R = int(off[-1])
+    k = values[0].shape[1]
+    out = np.zeros((R, k))
+    for ti in range(int(t.level_tile_off[-1])):
+        r0, r1 = int(t.tile_row[ti]), int(t.tile_row[ti + 1])
+        s0, s1 = int(t.tile_src[ti]), int(t.tile_src[ti + 1])
+        keys = t.src_list[s0:s1]
+        staged = np.stack([values[int(kk >> 30)][int(kk & 0x3FFFFFFF)] for kk in keys]) if s1 > s0 else None
+        for r in range(r0, r1):
+            lvl = int(np.searchsorted(off, r, side="right") - 1)
+            deg = int(t.eptr[r + 1] - t.eptr[r])
+            if heavy[lvl] >= 0 and deg > heavy[lvl]:
+                continue
+            if s1 == s0:  # direct tile
+                for (p, i, _), v in zip(segs, values):
+                    out[r] += v[i[p[r]:p[r + 1]]].sum(axis=0)
+            else:
+                out[r] = staged[t.lidx[t.eptr[r]:t.eptr[r + 1]]].sum(axis=0)
+    return out
+
+
+@pytest.mark.parametrize("max_staged,max_rows,max_entries", [(8, 64, 4096), (32, 16, 4096), (512, 1024, 4096), (64, 64, 30)])
+def test_tiles_reproduce_row_sums(max_staged, max_rows, max_entries):
+    rng = np.random.default_rng(7)
+    off = np.array([0, 40, 300, 420, 421, 500])
+    R = int(off[-1])
+    L = off.size - 1
+    segs = [(*_random_csr(rng, R, 300, 12, heavy_rows=[(50, 40), (420, 25)]), 300),
+            (*_random_csr(rng, R, 90, 3), 90)]
+    heavy = np.array([-1, 30, -1, 20, -1])
+    t = build_tiles(off, segs, heavy, max_staged=max_staged, max_rows=max_rows, max_entries=max_entries)
+    # tiles cover every row in order, levels start on tile boundaries
+    assert t.tile_row[0] == 0 and t.tile_row[-1] == R and np.all(np.diff(t.tile_row) > 0)
+    for l in range(L):
+        assert t.tile_row[t.level_tile_off[l]] == off[l] or off[l] == off[l + 1]
+    assert np.all(np.diff(t.tile_src) >= 0)
+    staged = np.diff(t.tile_src)
+    assert staged.max() <= max_staged
+    assert np.all(np.diff(t.tile_row) <= max_rows)
+    rows = np.diff(t.tile_row)
+    ents = t.eptr[t.tile_row[1:]] - t.eptr[t.tile_row[:-1]]
+    assert np.all(ents[staged > 0] <= max_entries)
+    for l in range(L):
+        lo, hi = t.level_tile_off[l], t.level_tile_off[l + 1]
+        st = staged[lo:hi] > 0
+        assert t.level_max_staged[l] == (staged[lo:hi].max() if hi > lo else 0)
+        assert t.level_max_rows[l] == (rows[lo:hi][st].max() if st.any() else 0)
+        assert t.level_max_entries[l] == (ents[lo:hi][st].max() if st.any() else 0)
+    # staged sources are distinct within a tile
+    for ti in range(int(t.level_tile_off[-1])):
+        keys = t.src_list[t.tile_src[ti]:t.tile_src[ti + 1]]
+        assert np.unique(keys).size == keys.size
+    values = [rng.standard_normal((300, 3)), rng.standard_normal((90, 3))]
+    want = np.zeros((R, 3))
+    for (p, i, _), v in zip(segs, values):
+        want += np.stack([v[i[p[r]:p[r + 1]]].sum(axis=0) for r in range(R)])
+    deg = np.diff(t.eptr)
+    lvl = np.searchsorted(off, np.arange(R), side="right") - 1
+    is_heavy = (heavy[lvl] >= 0) & (deg > heavy[lvl])
+    assert is_heavy[50] and is_heavy[420]
+    # heavy rows sit alone in direct tiles
+    for r in np.flatnonzero(is_heavy):
+        ti = int(np.searchsorted(t.tile_row, r, side="right") - 1)
+        assert t.tile_row[ti] == r and t.tile_row[ti + 1] == r + 1 and staged[ti] == 0
+    want[is_heavy] = 0.0
+    np.testing.assert_allclose(_tiled_sums(t, segs, values, off, heavy), want, rtol=1e-13, atol=1e-13)
+
+
+def test_long_rows_get_direct_tiles():
+    rng = np.random.default_rng(3)
+    off = np.array([0, 64])
+    p, i = _random_csr(rng, 64, 200, 4, heavy_rows=[(10, 100), (11, 120)])
+    t = build_tiles(off, [(p, i, 200)], np.array([-1]), max_staged=50, max_rows=1024)
+    starts = list(t.tile_row[:-1])
+    for r in (10, 11):
+        ti = starts.index(r)
+        assert t.tile_row[ti + 1] == r + 1 and t.tile_src[ti + 1] == t.tile_src[ti]
+
+
+def test_builder_rejects_bad_input():
+    off = np.array([0, 4])
+    p = np.array([0, 1, 2, 3, 4], dtype=np.int32)
+    i = np.array([0, 1, 2, 9], dtype=np.int32)
+    with pytest.raises(RuntimeError, match="out of range"):
+        build_tiles(off, [(p, i, 5)], np.array([-1]))
+    with pytest.raises(ValueError, match="cover every row"):
+        build_tiles(off, [(p[:3], i, 10)], np.array([-1]))
