@@ -278,7 +278,7 @@ __device__ __forceinline__ void store_row(const GrgHipGatherArgs& a, int64_t row
 // FP64 partial rows), a shuffle + LDS reduction, one store. Uses partial[] and
 // one barrier: callers looping over rows add a barrier between rows.
 template <typename T, int CPL, int U, typename Acc>
-__device__ __forceinline__ void workgroup_row(const GrgHipGatherArgs& a, int64_t row, int64_t h, int kl_shift,
+__device__ __forceinline__ void workgroup_row(const GrgHipGatherArgs& a, int64_t row, int64_t h, int by, int kl_shift,
                                               Acc (&partial)[kWavesPerBlock][64]) {
   const int KL = 1 << kl_shift;
   const int lane = threadIdx.x & 63;
@@ -286,7 +286,7 @@ __device__ __forceinline__ void workgroup_row(const GrgHipGatherArgs& a, int64_t
   const int cl = threadIdx.x & (KL - 1);
   const int e = threadIdx.x >> kl_shift;
   const int ES = kBlock >> kl_shift;
-  const int col = ((int)blockIdx.y * KL + cl) * CPL;
+  const int col = (by * KL + cl) * CPL;
   const bool live = col < a.k;
   Acc acc[CPL];
 #pragma unroll
@@ -313,11 +313,12 @@ __device__ __forceinline__ void workgroup_row(const GrgHipGatherArgs& a, int64_t
   if (wave == 0 && lane < KL && live) store_row<T, CPL>(a, row, col, acc);
 }
 
-template <typename T, int CPL, int U, int R>
-__global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherArgs a, int64_t n_light_blocks,
-                                                             int g_shift, int kl_shift) {
-  using Acc = typename SumOf<T>::type;
-  __shared__ Acc partial[kWavesPerBlock][64];
+// Block (bx, by) of a row-gather launch: a partials item, R-row lane groups of
+// the light path, or a heavy row. Shared by the one-launch kernel and the
+// job-table kernel.
+template <typename T, int CPL, int U, int R, typename Acc>
+__device__ __forceinline__ void gather_block(const GrgHipGatherArgs& a, int64_t bx, int by, int64_t n_light_blocks,
+                                             int g_shift, int kl_shift, Acc (&partial)[kWavesPerBlock][64]) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int KL = 1 << kl_shift;
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
 
   if (a.flags & GRG_HIP_FLAG_PARTIALS) {
     // ---- partials launch: workgroup i sums chunk j of split heavy row h ----
-    const int64_t item = blockIdx.x;
+    const int64_t item = bx;
     const int32_t h = a.split_row[item];
     const int64_t row = a.heavy_rows[h];
     const int64_t lo = (item - a.split_ptr[h]) * a.split_chunk;
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
     const int cl = threadIdx.x & (KL - 1);
     const int e = threadIdx.x >> kl_shift;
     const int ES = kBlock >> kl_shift;
-    const int col = ((int)blockIdx.y * KL + cl) * CPL;
+    const int col = (by * KL + cl) * CPL;
     if (col < a.k) {
       int64_t g = 0;  // offset of segment s in the concatenated entry list
       for (int s = 0; s < a.nseg; ++s) {
@@ -359,14 +360,14 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
     return;
   }
 
-  if ((int64_t)blockIdx.x < n_light_blocks) {
+  if (bx < n_light_blocks) {
     // ---- light path: R consecutive rows per G-lane group ----
     const int G = 1 << g_shift;
     const int lg = lane & (G - 1);
     const int e = lg >> kl_shift;
     const int ES = G >> kl_shift;
-    const int col = ((int)blockIdx.y * KL + (lg & (KL - 1))) * CPL;
-    int64_t blk = blockIdx.x;
+    const int col = (by * KL + (lg & (KL - 1))) * CPL;
+    int64_t blk = bx;
     if (a.flags & GRG_HIP_FLAG_XCD_REMAP) {
       // block b runs on XCD b % 8; give XCD x the x-th contiguous eighth of the rows
       const int64_t q = n_light_blocks >> 3, r = n_light_blocks & 7, x = blk & 7;
@@ -434,8 +435,16 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
   }
 
   // ---- heavy path: the whole workgroup on one row ----
-  const int64_t h = (int64_t)blockIdx.x - n_light_blocks;
-  workgroup_row<T, CPL, U>(a, a.heavy_rows[h], h, kl_shift, partial);
+  const int64_t h = bx - n_light_blocks;
+  workgroup_row<T, CPL, U>(a, a.heavy_rows[h], h, by, kl_shift, partial);
+}
+
+template <typename T, int CPL, int U, int R>
+__global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherArgs a, int64_t n_light_blocks,
+                                                             int g_shift, int kl_shift) {
+  using Acc = typename SumOf<T>::type;
+  __shared__ Acc partial[kWavesPerBlock][64];
+  gather_block<T, CPL, U, R>(a, (int64_t)blockIdx.x, (int)blockIdx.y, n_light_blocks, g_shift, kl_shift, partial);
 }
 
 // Padding (elements) after each staged row of a tile: at least 16 bytes and one
@@ -482,7 +491,7 @@ __global__ __launch_bounds__(kBlock) void tile_gather_kernel(const GrgHipGatherA
   const int64_t nt = a.num_tiles;
   int64_t t = blockIdx.x;
   if (t >= nt) {
-    workgroup_row<T, CPL, 8>(a, a.heavy_rows[t - nt], t - nt, kl_shift, partial);
+    workgroup_row<T, CPL, 8>(a, a.heavy_rows[t - nt], t - nt, (int)blockIdx.y, kl_shift, partial);
     return;
   }
   if (a.flags & GRG_HIP_FLAG_XCD_REMAP) {
@@ -496,7 +505,7 @@ __global__ __launch_bounds__(kBlock) void tile_gather_kernel(const GrgHipGatherA
     for (int32_t r = r0; r < r1; ++r) {
       const int32_t deg = a.eptr[r + 1] - a.eptr[r];
       if (a.heavy_threshold >= 0 && deg > a.heavy_threshold) continue;
-      workgroup_row<T, CPL, 8>(a, r, -1, kl_shift, partial);
+      workgroup_row<T, CPL, 8>(a, r, -1, (int)blockIdx.y, kl_shift, partial);
       __syncthreads();  // partial[] is reused by the next row
     }
     return;
@@ -612,6 +621,89 @@ __global__ __launch_bounds__(kBlock) void tile_gather_kernel(const GrgHipGatherA
       store_row<T, CPL>(a, r, col, acc);
     }
   }
+}
+
+// ---- job tables -------------------------------------------------------------
+
+constexpr uint32_t kSpinLimit = 1u << 24;  // barrier polls before a wait is declared failed
+
+// Grid-wide barrier of the persistent job kernel. Every wave first waits for
+// its own stores to reach L2 (s_waitcnt 0); one thread per workgroup then
+// writes the XCD's L2 back (agent-scope release), arrives on a monotonic
+// counter, polls it with relaxed loads (a poll must not invalidate caches that
+// working workgroups still use) and, once `target` workgroups have arrived,
+// drops stale L1/L2 lines with one agent-scope acquire. A wait is bounded: past
+// kSpinLimit polls it sets *status and gives up, so the kernel always ends.
+__device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t target, int32_t* status) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > kSpinLimit) {
+        __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+template <typename T, typename Acc>
+__device__ __forceinline__ void run_job_block(const GrgHipJob& J, int64_t bx, int by, Acc (&partial)[kWavesPerBlock][64]) {
+  const GrgHipGatherArgs& a = J.args;
+  if constexpr (sizeof(T) == 8) {
+    switch (a.cpl) {
+      case 1: gather_block<T, 1, 8, 1>(a, bx, by, J.n_light, J.g_shift, J.kl_shift, partial); break;
+      case 2: gather_block<T, 2, 8, 1>(a, bx, by, J.n_light, J.g_shift, J.kl_shift, partial); break;
+      default: gather_block<T, 4, 8, 1>(a, bx, by, J.n_light, J.g_shift, J.kl_shift, partial); break;
+    }
+  } else {
+    switch (a.cpl) {
+      case 1: gather_block<T, 1, 8, 1>(a, bx, by, J.n_light, J.g_shift, J.kl_shift, partial); break;
+      case 2: gather_block<T, 2, 8, 1>(a, bx, by, J.n_light, J.g_shift, J.kl_shift, partial); break;
+      case 8: gather_block<T, 8, 8, 1>(a, bx, by, J.n_light, J.g_shift, J.kl_shift, partial); break;
+      default: gather_block<T, 4, 8, 1>(a, bx, by, J.n_light, J.g_shift, J.kl_shift, partial); break;
+    }
+  }
+}
+
+// Phases [phase_begin, phase_end) of a job table. Each phase's virtual
+// workgroups (jobs laid end to end, each job gx x gy) are taken in strides of
+// the grid; with a barrier counter (persistent launch) the grid meets between
+// phases, without one the launch covers a single phase.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void jobs_kernel(const GrgHipJob* __restrict__ jobs,
+                                                      const int32_t* __restrict__ phase_job_off,
+                                                      const int64_t* __restrict__ phase_blocks, int32_t phase_begin,
+                                                      int32_t phase_end, uint32_t* barrier, int32_t* status) {
+  using Acc = typename SumOf<T>::type;
+  __shared__ Acc partial[kWavesPerBlock][64];
+  for (int32_t p = phase_begin; p < phase_end; ++p) {
+    const int32_t j0 = phase_job_off[p], j1 = phase_job_off[p + 1];
+    const int64_t nb = phase_blocks[p];
+    for (int64_t vb = blockIdx.x; vb < nb; vb += gridDim.x) {
+      int32_t lo = j0, hi = j1 - 1;  // the last job of the phase starting at or before vb
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].block_begin <= vb) lo = mid;
+        else hi = mid - 1;
+      }
+      const GrgHipJob& J = jobs[lo];
+      const int64_t local = vb - J.block_begin;
+      run_job_block<T>(J, local % J.gx, (int)(local / J.gx), partial);
+      __syncthreads();  // partial[] is reused by the next block
+    }
+    if (barrier && p + 1 < phase_end) grid_barrier(barrier, (uint32_t)(p - phase_begin + 1) * gridDim.x, status);
+  }
+}
+
+__global__ void reset_counter_kernel(uint32_t* bar) {
+  if (threadIdx.x == 0) *bar = 0u;
 }
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
@@ -783,9 +875,117 @@ int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
   return 0;
 }
 
+// Grid of one row-gather launch (as launch_one computes it).
+int launch_grid(const GrgHipGatherArgs& a, int64_t* gx, int64_t* gy, int64_t* n_light, int* g_shift, int* kl_shift) {
+  const int G = a.kl * a.es;
+  *g_shift = ilog2(G);
+  *kl_shift = ilog2(a.kl);
+  *n_light = 0;
+  if (a.flags & GRG_HIP_FLAG_PARTIALS) {
+    *gx = a.split_items;
+  } else {
+    const int64_t rows_per_block = (int64_t)kWavesPerBlock * (64 / G);  // one row per lane group in job kernels
+    *n_light = (a.row_end - a.row_begin + rows_per_block - 1) / rows_per_block;
+    *gx = *n_light + a.num_heavy;
+  }
+  const int chunk = a.kl * a.cpl;
+  *gy = (a.k + chunk - 1) / chunk;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int grg_hip_jobs_build(const GrgHipGatherArgs* args, const int32_t* phase_of, int32_t njobs, int32_t nphase,
+                       GrgHipJob* jobs, int32_t* phase_job_off, int64_t* phase_blocks) {
+  g_last_error.clear();
+  if (njobs < 0 || nphase < 0 || (njobs > 0 && (!args || !phase_of || !jobs)) || !phase_job_off || (nphase > 0 && !phase_blocks))
+    return fail("bad job table arguments");
+  for (int32_t p = 0; p <= nphase; ++p) phase_job_off[p] = 0;
+  for (int32_t p = 0; p < nphase; ++p) phase_blocks[p] = 0;
+  int32_t cur = -1;
+  for (int32_t i = 0; i < njobs; ++i) {
+    const GrgHipGatherArgs& a = args[i];
+    if (int rc = validate(a)) {
+      g_last_error = "job " + std::to_string(i) + ": " + g_last_error;
+      return rc;
+    }
+    if (a.flags & GRG_HIP_FLAG_TILED) return fail("job " + std::to_string(i) + ": tiled launches are not run from job tables");
+    if (a.dtype != args[0].dtype) return fail("jobs of one table share one value type");
+    const int32_t p = phase_of[i];
+    if (p < 0 || p >= nphase || p < cur) return fail("phase_of must be non-decreasing and in [0, nphase)");
+    while (cur < p) phase_job_off[++cur] = i;
+    GrgHipJob& J = jobs[i];
+    J.args = a;
+    int g_shift = 0, kl_shift = 0;
+    launch_grid(a, &J.gx, &J.gy, &J.n_light, &g_shift, &kl_shift);
+    J.g_shift = g_shift;
+    J.kl_shift = kl_shift;
+    J.block_begin = phase_blocks[p];
+    phase_blocks[p] += J.gx * J.gy;
+    if (phase_blocks[p] > 0x7fffffffLL) return fail("phase " + std::to_string(p) + " has too many workgroups");
+  }
+  while (cur < nphase) phase_job_off[++cur] = njobs;
+  return 0;
+}
+
+int grg_hip_jobs_max_grid(int32_t dtype, void* stream, int32_t* max_workgroups) {
+  g_last_error.clear();
+  if (!max_workgroups) return fail("max_workgroups is null");
+  int dev = 0;
+  hipError_t err = hipStreamGetDevice(static_cast<hipStream_t>(stream), &dev);
+  if (err != hipSuccess) return fail(std::string("hipStreamGetDevice: ") + hipGetErrorString(err), 2);
+  int cus = 0, per_cu = 0;
+  err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (err != hipSuccess) return fail(std::string("hipDeviceGetAttribute: ") + hipGetErrorString(err), 2);
+  if (dtype == GRG_HIP_F64)
+    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&jobs_kernel<double>), kBlock, 0);
+  else
+    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&jobs_kernel<float>), kBlock, 0);
+  if (err != hipSuccess) return fail(std::string("hipOccupancyMaxActiveBlocksPerMultiprocessor: ") + hipGetErrorString(err), 2);
+  *max_workgroups = cus * per_cu;
+  return 0;
+}
+
+int grg_hip_jobs_run(const GrgHipJob* jobs, const int32_t* phase_job_off, const int64_t* phase_blocks,
+                     const int64_t* phase_blocks_host, int32_t dtype, int32_t nphase, int32_t persistent,
+                     int32_t grid, uint32_t* barrier, int32_t* status, void* stream) {
+  g_last_error.clear();
+  if (dtype != GRG_HIP_F64 && dtype != GRG_HIP_F32C) return fail("unknown dtype code " + std::to_string(dtype));
+  if (nphase < 0 || (nphase > 0 && (!jobs || !phase_job_off || !phase_blocks || !phase_blocks_host)))
+    return fail("bad job table arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (persistent) {
+    if (!barrier || !status) return fail("a persistent run needs the barrier counter and the status word");
+    int32_t max_grid = 0;
+    if (int rc = grg_hip_jobs_max_grid(dtype, stream, &max_grid)) return rc;
+    if (grid < 1 || grid > max_grid)
+      return fail("persistent grid " + std::to_string(grid) + " outside [1, " + std::to_string(max_grid) + "]");
+    if (nphase == 0) return 0;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, barrier);
+    if (dtype == GRG_HIP_F64)
+      hipLaunchKernelGGL(jobs_kernel<double>, dim3(grid), dim3(kBlock), 0, s, jobs, phase_job_off, phase_blocks, 0, nphase, barrier, status);
+    else
+      hipLaunchKernelGGL(jobs_kernel<float>, dim3(grid), dim3(kBlock), 0, s, jobs, phase_job_off, phase_blocks, 0, nphase, barrier, status);
+  } else {
+    for (int32_t p = 0; p < nphase; ++p) {
+      const int64_t nb = phase_blocks_host[p];
+      if (nb <= 0) continue;
+      if (nb > 0x7fffffffLL) return fail("phase has too many workgroups");
+      if (dtype == GRG_HIP_F64)
+        hipLaunchKernelGGL(jobs_kernel<double>, dim3((unsigned)nb), dim3(kBlock), 0, s, jobs, phase_job_off, phase_blocks, p, p + 1,
+                           (uint32_t*)nullptr, (int32_t*)nullptr);
+      else
+        hipLaunchKernelGGL(jobs_kernel<float>, dim3((unsigned)nb), dim3(kBlock), 0, s, jobs, phase_job_off, phase_blocks, p, p + 1,
+                           (uint32_t*)nullptr, (int32_t*)nullptr);
+    }
+  }
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return fail(std::string("kernel launch failed: ") + hipGetErrorString(err), 2);
+  return 0;
+}
+
 
 int grg_hip_gather(const GrgHipGatherArgs* args, void* stream) {
   if (!args) return fail("args is null");

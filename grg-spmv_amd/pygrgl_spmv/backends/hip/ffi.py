@@ -13,7 +13,7 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 7
+ABI_VERSION = 8
 F64 = 0
 F32C = 1
 FLAG_XCD_REMAP = 2
@@ -78,6 +78,11 @@ class GrgHipGatherArgs(ctypes.Structure):
     ]
 
 
+class GrgHipJob(ctypes.Structure):
+    _fields_ = [("args", GrgHipGatherArgs), ("block_begin", c_int64), ("gx", c_int64), ("gy", c_int64), ("n_light", c_int64),
+                ("g_shift", c_int32), ("kl_shift", c_int32)]
+
+
 class HipLibrary:
     def __init__(self, path: Path = LIB_PATH):
         if not Path(path).exists():
@@ -99,6 +104,14 @@ class HipLibrary:
         lib.grg_hip_stream_device.restype = ctypes.c_int
         lib.grg_hip_stream_capturing.argtypes = [c_void_p, ctypes.POINTER(c_int32)]
         lib.grg_hip_stream_capturing.restype = ctypes.c_int
+        lib.grg_hip_jobs_build.argtypes = [ctypes.POINTER(GrgHipGatherArgs), c_void_p, c_int32, c_int32,
+                                           ctypes.POINTER(GrgHipJob), c_void_p, c_void_p]
+        lib.grg_hip_jobs_build.restype = ctypes.c_int
+        lib.grg_hip_jobs_max_grid.argtypes = [c_int32, c_void_p, ctypes.POINTER(c_int32)]
+        lib.grg_hip_jobs_max_grid.restype = ctypes.c_int
+        lib.grg_hip_jobs_run.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32,
+                                         c_void_p, c_void_p, c_void_p]
+        lib.grg_hip_jobs_run.restype = ctypes.c_int
         version = int(lib.grg_hip_abi_version())
         if version != ABI_VERSION:
             raise RuntimeError(f"libgrg_hip.so ABI version {version} != expected {ABI_VERSION}; rebuild the extension")
@@ -115,6 +128,34 @@ class HipLibrary:
 
     def gather_batch(self, array, n: int, stream_ptr: int) -> None:
         self._check(self.lib.grg_hip_gather_batch(array, c_int32(int(n)), c_void_p(int(stream_ptr))), "grg_hip_gather_batch")
+
+    def jobs_build(self, args_list, phase_of):
+        """Job table of launch descriptors (grg_hip_jobs_build): (jobs ctypes array,
+        phase_job_off int32[nphase + 1], phase_blocks int64[nphase])."""
+        import numpy as np
+
+        n = len(args_list)
+        phase = np.ascontiguousarray(phase_of, dtype=np.int32)
+        nphase = int(phase.max()) + 1 if n else 0
+        arr = (GrgHipGatherArgs * max(n, 1))(*args_list)
+        jobs = (GrgHipJob * max(n, 1))()
+        job_off = np.zeros(nphase + 1, dtype=np.int32)
+        blocks = np.zeros(max(nphase, 1), dtype=np.int64)
+        self._check(self.lib.grg_hip_jobs_build(arr, phase.ctypes.data, n, nphase, jobs, job_off.ctypes.data, blocks.ctypes.data),
+                    "grg_hip_jobs_build")
+        return jobs, job_off, blocks[:nphase]
+
+    def jobs_max_grid(self, dtype: int, stream_ptr: int) -> int:
+        out = c_int32(0)
+        self._check(self.lib.grg_hip_jobs_max_grid(int(dtype), c_void_p(int(stream_ptr)), ctypes.byref(out)), "grg_hip_jobs_max_grid")
+        return int(out.value)
+
+    def jobs_run(self, jobs_dev: int, job_off_dev: int, blocks_dev: int, blocks_host, dtype: int, nphase: int,
+                 persistent: bool, grid: int, barrier_dev: int, status_dev: int, stream_ptr: int) -> None:
+        self._check(self.lib.grg_hip_jobs_run(c_void_p(int(jobs_dev)), c_void_p(int(job_off_dev)), c_void_p(int(blocks_dev)),
+                                              c_void_p(blocks_host.ctypes.data), int(dtype), int(nphase), int(bool(persistent)),
+                                              int(grid), c_void_p(int(barrier_dev)), c_void_p(int(status_dev)),
+                                              c_void_p(int(stream_ptr))), "grg_hip_jobs_run")
 
     def device_count(self) -> int:
         out = c_int32(0)
@@ -142,4 +183,4 @@ def hip_library() -> HipLibrary:
     return _LIB
 
 
-__all__ = ["ABI_VERSION", "F32C", "F64", "GrgHipGatherArgs", "GrgHipMap", "GrgHipSeg", "HipLibrary", "LIB_PATH", "hip_library"]
+__all__ = ["ABI_VERSION", "F32C", "F64", "GrgHipGatherArgs", "GrgHipJob", "GrgHipMap", "GrgHipSeg", "HipLibrary", "LIB_PATH", "hip_library"]

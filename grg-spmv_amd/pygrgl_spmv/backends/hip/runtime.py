@@ -200,6 +200,21 @@ class _HipArtifact:
     tiles: dict = field(default_factory=dict)         # config -> per-level tile offsets and LDS maxima
 
 
+@dataclass
+class _Buffers:
+    """Value buffers a prepared sweep writes: node state (rows of ``ld_state``
+    values), the dense arenas ``io0`` / ``io1`` / ``aux`` (2-D, k columns at most)
+    and the FP64 partial rows of split rows. The runtime's own arena serves the
+    ordinary ops; a batched sweep gives every GRG a private set."""
+
+    state: object
+    ld_state: int
+    io0: object
+    io1: object
+    aux: object
+    partials: object
+
+
 def _i32(values, label: str) -> np.ndarray:
     arr = np.asarray(values)
     out = np.empty(arr.shape, dtype=np.int32)
@@ -308,6 +323,119 @@ class _HipPreparedMatmul:
 
     def _call_prelocked(self) -> None:
         self._runtime._execute(self._artifact, self._spec, self._steps)
+
+
+class HipBatchedSweep:
+    """Several sweeps (typically one per GRG of ``runtime.grgs``, each with its own k)
+    run together from one job table: phase p holds the p-th launch of every sweep,
+    so a level of all GRGs is one kernel (``persistent=False``: one launch per
+    phase) or the whole batch is ONE persistent launch whose workgroups meet at a
+    grid-wide barrier between phases (``persistent=True``).
+
+    No reference counterpart: the reference runs ``runtime.grgs`` one GRG after the
+    other. Every sweep writes private buffers, so ``entries[i]`` has its own
+    ``inputs[i]`` / ``outputs[i]`` (and ``miss_inputs[i]``, ``init_vectors[i]``...)
+    views, laid out as the prepared op's (k x cols, transposed views of row-major
+    arenas). Needs every block resident; tiled level launches run in their per-row
+    form inside job tables. Calling the object runs the batch on the runtime's
+    caller stream (capturable into a hipGraph); ``check()`` synchronizes and
+    raises if a persistent barrier wait ran past its bound."""
+
+    def __init__(self, runtime: "HipRuntime", entries, *, persistent: bool, grid: int | None) -> None:
+        torch = _torch()
+        self._runtime = runtime
+        self.persistent = bool(persistent)
+        self.specs = []
+        self.inputs, self.outputs, self.miss_inputs, self.miss_outputs = [], [], [], []
+        self.init_vectors, self.init_matrices = [], []
+        self._buffers = []
+        launches, phase_of = [], []
+        dev = runtime.device
+        tdt = runtime._torch_dtype()
+        items_max = max([a.max_items for a in runtime._artifacts] + [1])
+        with torch.cuda.device(dev):
+            for e in entries:
+                e = dict(e)
+                grg = e.pop("grg")
+                spec = grg._prepare_cuda_spec(direction=e.pop("direction"), k=int(e.pop("k")),
+                                              emit_all_nodes=bool(e.pop("emit_all_nodes", False)),
+                                              by_individual=bool(e.pop("by_individual", False)),
+                                              init_mode=e.pop("init_mode", "none"), use_miss=bool(e.pop("use_miss", False)))
+                if e:
+                    raise TypeError(f"unknown batched sweep entry keys: {sorted(e)}")
+                if spec.direction == Direction.UP and runtime.layout.pair.plan_up is None:
+                    raise ValueError("UP plan is not configured")
+                if spec.direction == Direction.DOWN and runtime.layout.pair.plan_down is None:
+                    raise ValueError("DOWN plan is not configured")
+                art = runtime._artifacts[int(grg._artifact_index)]
+                st = art.state
+                k = int(spec.k)
+                io0_rows = max(int(spec.input_cols), int(spec.output_cols), 1)
+                io1_rows = 0
+                if spec.use_miss:
+                    io1_rows = st.num_mutations
+                if spec.init_mode == InitMode.VECTOR:
+                    io1_rows = max(io1_rows, 1)
+                if spec.init_mode == InitMode.MATRIX:
+                    io1_rows = max(io1_rows, st.num_nodes)
+                bufs = _Buffers(state=torch.zeros((st.num_nodes * k,), dtype=tdt, device=dev), ld_state=k,
+                                io0=torch.zeros((io0_rows, k), dtype=tdt, device=dev),
+                                io1=torch.zeros((io1_rows, k), dtype=tdt, device=dev) if io1_rows else None,
+                                aux=torch.zeros((st.num_nodes, k), dtype=tdt, device=dev) if spec.emit_all_nodes else None,
+                                partials=torch.zeros((items_max * k,), dtype=torch.float64, device=dev))
+                steps = runtime._build_steps(art, spec, bufs)
+                if any(it[0] != "batch" for it in steps):
+                    raise ValueError("a batched sweep needs every block of its GRG resident")
+                p = 0
+                for _, arr, n in steps:
+                    for i in range(n):
+                        a = ffi.GrgHipGatherArgs()
+                        ctypes.pointer(a)[0] = arr[i]
+                        a.flags &= ~ffi.FLAG_TILED
+                        launches.append(a)
+                        phase_of.append(p)
+                        p += 1
+                self.specs.append(spec)
+                self._buffers.append(bufs)
+                self.inputs.append(bufs.io0[: spec.input_cols, :k].T)
+                out_internal = bufs.aux[: st.num_nodes, :k] if spec.emit_all_nodes else bufs.io0[: spec.output_cols, :k]
+                self.outputs.append(out_internal.T)
+                miss = bufs.io1[: st.num_mutations, :k].T if spec.use_miss else None
+                self.miss_inputs.append(miss if spec.use_miss and spec.direction == Direction.DOWN else None)
+                self.miss_outputs.append(miss if spec.use_miss and spec.direction == Direction.UP else None)
+                self.init_vectors.append(bufs.io1[0, :k] if spec.init_mode == InitMode.VECTOR else None)
+                self.init_matrices.append(bufs.io1[: st.num_nodes, :k].T if spec.init_mode == InitMode.MATRIX else None)
+            order = sorted(range(len(launches)), key=lambda i: phase_of[i])
+            jobs, job_off, blocks = runtime._lib.jobs_build([launches[i] for i in order], [phase_of[i] for i in order])
+            self.num_jobs = len(launches)
+            self.num_phases = int(blocks.size)
+            raw = np.frombuffer(bytes(jobs), dtype=np.uint8)
+            self._jobs_dev = torch.from_numpy(raw.copy()).to(dev)
+            self._job_off_dev = torch.from_numpy(job_off).to(dev)
+            self._blocks_dev = torch.from_numpy(blocks.copy() if blocks.size else np.zeros(1, np.int64)).to(dev)
+            self._blocks_host = np.ascontiguousarray(blocks if blocks.size else np.zeros(1, np.int64))
+            self._barrier = torch.zeros((1,), dtype=torch.int32, device=dev)
+            self._status = torch.zeros((1,), dtype=torch.int32, device=dev)
+            max_grid = runtime._lib.jobs_max_grid(runtime._dtype_code, int(runtime._root_stream.cuda_stream))
+            want = int(self._blocks_host.max()) if self.num_phases else 1
+            self.grid = max(1, min(max_grid, want if grid is None else int(grid)))
+            if grid is not None and not 1 <= int(grid) <= max_grid:
+                raise ValueError(f"grid must be in [1, {max_grid}] (co-resident workgroups of the persistent kernel)")
+        self.max_grid = max_grid
+
+    def __call__(self) -> None:
+        rt = self._runtime
+        with rt._call_scope():
+            with rt._caller_root_scope():
+                rt._lib.jobs_run(self._jobs_dev.data_ptr(), self._job_off_dev.data_ptr(), self._blocks_dev.data_ptr(),
+                                 self._blocks_host, rt._dtype_code, self.num_phases, self.persistent, self.grid,
+                                 self._barrier.data_ptr(), self._status.data_ptr(), int(rt._root_stream.cuda_stream))
+
+    def check(self) -> None:
+        """Synchronize; raise if any persistent barrier wait ran past its bound."""
+        if int(self._status.item()) != 0:
+            self._status.zero_()
+            raise RuntimeError("HIP batched sweep: a grid barrier wait ran past its bound; the results are invalid")
 
 
 class HipRuntime:
@@ -478,6 +606,17 @@ class HipRuntime:
             yield self._sweep_timer
         finally:
             self._sweep_timer = None
+
+    def prepare_batched_sweep(self, entries, *, persistent: bool = True, grid: int | None = None) -> HipBatchedSweep:
+        """Sweeps of several GRGs run from one job table (see ``HipBatchedSweep``).
+        ``entries``: dicts with ``grg`` (one of ``self.grgs``), ``direction``, ``k``
+        and optionally ``emit_all_nodes``, ``by_individual``, ``init_mode``,
+        ``use_miss`` as for ``prepare_matmul_cuda``. ``grid``: workgroups of the
+        persistent launch (default: as many as are resident at once, at most the
+        largest phase)."""
+        if not self._entered:
+            raise RuntimeError("HipRuntime must be entered before preparing a batched sweep")
+        return HipBatchedSweep(self, entries, persistent=persistent, grid=grid)
 
     def _prepare_matmul_cuda(self, grg: BoundGRG, spec: _CudaMatmulSpec) -> _HipPreparedMatmul:
         if spec.direction == Direction.UP and self.layout.pair.plan_up is None:
@@ -738,7 +877,7 @@ class HipRuntime:
 
     def _args(self, *, rows, k, out, ld_out, segs=(), maps=(), row_scale=None, col_vec=None, accumulate=False,
               sched: _LevelSchedule | None = None, sched_base: int = 0, mean_degree: float = 1.0,
-              out_base=0, tiles: dict | None = None) -> ffi.GrgHipGatherArgs:
+              out_base=0, tiles: dict | None = None, partials=None) -> ffi.GrgHipGatherArgs:
         """Launch descriptor for rows [rows); ``sched`` adds the level's heavy rows,
         ``tiles`` (from ``_tile_fields``) the level's tile tables."""
         a = ffi.GrgHipGatherArgs()
@@ -754,7 +893,7 @@ class HipRuntime:
                 a.split_ptr = sched_base + 4 * sched.split_ptr_offset
                 a.split_row = sched_base + 4 * sched.split_row_offset
                 a.split_items, a.split_chunk = sched.num_items, sched.split_chunk
-                a.partials = self._partials.data_ptr()
+                a.partials = (self._partials if partials is None else partials).data_ptr()
         else:
             a.heavy_threshold, a.heavy_rows, a.num_heavy = -1, 0, 0
         a.flags = 0
@@ -858,19 +997,27 @@ class HipRuntime:
         if best is not None:
             a.cpl, a.kl, a.es, a.unroll, a.rows_per_group, a.flags = best
 
-    def _build_steps(self, art: _HipArtifact, spec: _CudaMatmulSpec):
-        """The prepared sweep: a list of ("batch", array, n) and ("stream", step, args, events) items."""
+    def _arena(self) -> _Buffers:
+        return _Buffers(state=self._state, ld_state=0, io0=self._io0, io1=self._io1, aux=self._aux, partials=self._partials)
+
+    def _build_steps(self, art: _HipArtifact, spec: _CudaMatmulSpec, bufs: _Buffers | None = None):
+        """The prepared sweep: a list of ("batch", array, n) and ("stream", step, args, events) items,
+        writing into ``bufs`` (the runtime's arena by default)."""
+        bufs = self._arena() if bufs is None else bufs
         D = art.dev
         st = art.state
         off = art.level_offsets
         L = off.size - 1
         k = int(spec.k)
         w = self._elem()
-        V = self._state.data_ptr()
+        V = bufs.state.data_ptr()
         ldv = k
-        max_k = int(self._io0.shape[1])
-        io0 = self._io0.data_ptr()
-        io1 = self._io1.data_ptr() if self._io1 is not None else 0
+        max_k = int(bufs.io0.shape[1])
+        io0 = bufs.io0.data_ptr()
+        io1 = bufs.io1.data_ptr() if bufs.io1 is not None else 0
+        aux = bufs.aux.data_ptr() if bufs.aux is not None else 0
+        max_k_aux = int(bufs.aux.shape[1]) if bufs.aux is not None else max_k
+        partials = bufs.partials
         sched_base = D["sched"].data_ptr()
 
         # init terms folded into the sweep (backend_init_mode, as reference grg/__init__.py:498-502)
@@ -919,7 +1066,7 @@ class HipRuntime:
             sched = art.sched[(cfg, l)] if segs else None
             level_args = self._args(rows=(lo, hi), k=k, out=V, ld_out=ldv, segs=segs, maps=maps,
                                     row_scale=row_scale_init, col_vec=col_vec_init, sched=sched, sched_base=sched_base,
-                                    tiles=self._tile_fields(art, cfg, l) if segs else None)
+                                    tiles=self._tile_fields(art, cfg, l) if segs else None, partials=partials)
             if segs:
                 self._tune(level_args, (art.path, cfg, l, k))
             if sched is not None and sched.num_items:
@@ -940,7 +1087,7 @@ class HipRuntime:
                 items.append(("stream", step, args, events[l][i], (l, i)))
         # write-out
         if spec.emit_all_nodes:
-            batch.append(self._args(rows=(0, st.num_nodes), k=k, out=self._aux.data_ptr(), ld_out=max_k,
+            batch.append(self._args(rows=(0, st.num_nodes), k=k, out=aux, ld_out=max_k_aux,
                                     maps=[(D["inv_perm"].data_ptr(), V, ldv, 0, st.num_nodes)]))
         elif spec.direction == Direction.UP:
             rs = cv = None
@@ -1023,4 +1170,4 @@ class HipRuntime:
                 timer.append((t_start, t_end))
 
 
-__all__ = ["HipRuntime", "launch_shape"]
+__all__ = ["HipBatchedSweep", "HipRuntime", "launch_shape"]

@@ -9,6 +9,10 @@ reference's multi-GRG runtime, ``triton/backend.py:373-410``), mixed column
 counts k in [1, 64] per GRG. Every GRG's up and down op (input copy, sweep,
 output copy) is recorded into ONE hipGraph when every block is resident; the
 graph is replayed and each replay checked bit for bit against the eager run.
+The same sweeps then run from job tables (``HipRuntime.prepare_batched_sweep``):
+level p of every GRG in one phase, as ONE persistent launch per direction with
+grid-wide barriers between phases, and as one launch per phase; both are
+checked against the per-GRG ops, captured and replayed.
 (Parity of multi-GRG runtimes with the CPU oracle is covered by the tests.)
 
 Prints progress on stderr and one JSON line on stdout.
@@ -122,6 +126,67 @@ def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: i
             out["graph_edge_updates_per_s"] = edges / (min(times) / 1e3)
         log(f"eager {out['eager_ms']:.2f} ms" + (f", graph replay {out['graph_replay_ms']:.2f} ms" if capturable else
                                                   ", not captured (streamed blocks)"))
+        if capturable:
+            # the same 44 sweeps from job tables: all GRGs' level p in one phase
+            tol = 1e-12 if np.dtype(dtype) == np.float64 else 1e-5
+            for mode, persistent in (("persistent", True), ("grouped", False)):
+                ups = rt.prepare_batched_sweep([dict(grg=g, direction="up", k=mixed_k(i)) for i, g in enumerate(grgs)],
+                                               persistent=persistent)
+                dns = rt.prepare_batched_sweep([dict(grg=g, direction="down", k=mixed_k(i)) for i, g in enumerate(grgs)],
+                                               persistent=persistent)
+
+                def batched():
+                    for v, x in zip(ups.inputs, xs):
+                        v.copy_(x)
+                    for v, y in zip(dns.inputs, ys):
+                        v.copy_(y)
+                    ups()
+                    dns()
+
+                with torch.cuda.stream(cap):
+                    batched()
+                torch.cuda.synchronize()
+                ups.check()
+                dns.check()
+                err = 0.0
+                for a, b in zip((*ups.outputs, *dns.outputs), (*eager_up, *eager_dn)):
+                    a64 = a.cpu().double()
+                    b64 = b.double()
+                    err = max(err, float((a64 - b64).norm() / max(float(b64.norm()), 1e-300)))
+                if err > tol:
+                    raise AssertionError(f"{mode} batched sweep differs from the per-GRG ops: {err:.2e}")
+                with torch.cuda.stream(cap):
+                    ev[0].record(cap)
+                    batched()
+                    ev[1].record(cap)
+                torch.cuda.synchronize()
+                res = {"phases_up": ups.num_phases, "phases_down": dns.num_phases, "jobs": ups.num_jobs + dns.num_jobs,
+                       "grid": ups.grid if persistent else None, "eager_ms": ev[0].elapsed_time(ev[1]),
+                       "max_rel_err_vs_per_grg_ops": err}
+                eager_b = [o.cpu().clone() for o in (*ups.outputs, *dns.outputs)]
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph, stream=cap):
+                    batched()
+                times = []
+                for rep in range(int(replays)):
+                    for o in (*ups.outputs, *dns.outputs):
+                        o.zero_()
+                    torch.cuda.synchronize()
+                    with torch.cuda.stream(cap):
+                        ev[0].record(cap)
+                        graph.replay()
+                        ev[1].record(cap)
+                    torch.cuda.synchronize()
+                    times.append(ev[0].elapsed_time(ev[1]))
+                    ups.check()
+                    if not all(torch.equal(o.cpu(), b) for o, b in zip((*ups.outputs, *dns.outputs), eager_b)):
+                        raise AssertionError(f"{mode} batched graph replay {rep} differs from its eager run")
+                res["graph_replay_ms"] = min(times)
+                res["graph_edge_updates_per_s"] = edges / (min(times) / 1e3)
+                out[f"batched_{mode}"] = res
+                log(f"batched ({mode}): {res['jobs']} jobs in {res['phases_up']} + {res['phases_down']} phases, eager "
+                    f"{res['eager_ms']:.2f} ms, graph replay {res['graph_replay_ms']:.2f} ms, rel err vs per-GRG ops {err:.1e}")
+                del graph, ups, dns
     out["seconds_total"] = time.perf_counter() - t0
     return out
 

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 7
+#define GRG_HIP_ABI_VERSION 8
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -160,6 +160,51 @@ int grg_hip_gather(const GrgHipGatherArgs* args, void* stream);
  * seed-fused levels then write-out, with the partials launch of a level's
  * split rows right before the level), from one host call. */
 int grg_hip_gather_batch(const GrgHipGatherArgs* args, int32_t n, void* stream);
+
+/* ---- Job tables: many row-gather launches in one kernel -------------------
+ *
+ * No reference counterpart. The reference runs runtime.grgs one GRG after the
+ * other, one launch per block (triton/backend.py:970-1049); here the launches
+ * of several sweeps (one per GRG, each with its own k and buffers) are laid
+ * out as a table of jobs in phases: phase p holds the p-th launch of every
+ * sweep, so one kernel runs a level of all GRGs at once. Jobs run the light
+ * path with 8 gathers in flight and one row per lane group (the unroll and
+ * rows_per_group of their descriptors are ignored); tiled descriptors are not
+ * accepted. */
+typedef struct GrgHipJob {
+  GrgHipGatherArgs args;
+  int64_t block_begin;  /* first virtual workgroup of the job within its phase */
+  int64_t gx;           /* workgroups along the rows: light blocks + heavy rows, or partial items */
+  int64_t gy;           /* column chunks */
+  int64_t n_light;      /* light-path workgroups */
+  int32_t g_shift;
+  int32_t kl_shift;
+} GrgHipJob;
+
+/* Validate args[0..njobs) (one value type; phase_of non-decreasing in
+ * [0, nphase)) and fill the host arrays jobs[njobs], phase_job_off[nphase + 1]
+ * (phase p = jobs [phase_job_off[p], phase_job_off[p + 1])) and
+ * phase_blocks[nphase] (virtual workgroups of phase p). The caller copies the
+ * three arrays to device memory for grg_hip_jobs_run. */
+int grg_hip_jobs_build(const GrgHipGatherArgs* args, const int32_t* phase_of, int32_t njobs, int32_t nphase,
+                       GrgHipJob* jobs, int32_t* phase_job_off, int64_t* phase_blocks);
+
+/* Workgroups of the persistent job kernel that are resident at once on the
+ * stream's device (occupancy x compute units): the largest persistent grid. */
+int grg_hip_jobs_max_grid(int32_t dtype, void* stream, int32_t* max_workgroups);
+
+/* Run all phases of a job table (device copies of the arrays filled by
+ * grg_hip_jobs_build; phase_blocks_host is the host copy of phase_blocks).
+ * persistent = 0: one launch per phase, in stream order. persistent = 1: ONE
+ * launch of `grid` workgroups (1 <= grid <= grg_hip_jobs_max_grid) that loop
+ * over each phase's workgroups and meet at a grid-wide barrier between
+ * phases; `barrier` (one uint32 of device memory) is reset by a kernel on the
+ * stream before the launch, and `status` (one int32 of device memory, set
+ * to 0 by the caller) becomes 1 if a barrier wait ran past its bound (the
+ * results are then invalid). Both are capturable into a hipGraph. */
+int grg_hip_jobs_run(const GrgHipJob* jobs, const int32_t* phase_job_off, const int64_t* phase_blocks,
+                     const int64_t* phase_blocks_host, int32_t dtype, int32_t nphase, int32_t persistent,
+                     int32_t grid, uint32_t* barrier, int32_t* status, void* stream);
 
 /* Thread-local description of the last failure ("" if none). */
 const char* grg_hip_last_error(void);

@@ -395,6 +395,107 @@ def test_multi_grg_mixed_k(artifact_dir, art_a, art_b):
                 _check(grg.matmul(x, direction), O.matmul(g, x, direction))
 
 
+@pytest.mark.parametrize("persistent", [True, False])
+def test_batched_sweep_of_several_grgs(artifact_dir, art_a, art_b, persistent):
+    """Up and down sweeps of three GRGs with mixed k (missing data, by-individual,
+    xtx init among them) from one job table: one persistent launch with grid-wide
+    barriers, or one launch per phase; every output against the oracle, and a
+    capped persistent grid (workgroups looping over each phase) gives the same."""
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    band = write_band_artifact(artifact_dir, "band-n64-batch", n=64, bandwidth=8)
+    arts = [art_a, art_b, band]
+    oracles = [O.load_artifact(p) for p in arts]
+    layout = build_hip_layout(arts, requirements=full_requirements(max_k_up=16, max_k_down=16))
+    rng = np.random.default_rng(3850)
+    with HipRuntime(layout) as rt:
+        ga, gb, gc = rt.grgs
+        oa, ob, oc = oracles
+        entries = [
+            dict(grg=ga, direction="up", k=5), dict(grg=ga, direction="down", k=16, use_miss=True),
+            dict(grg=gb, direction="up", k=1), dict(grg=gb, direction="down", k=7),
+            dict(grg=gc, direction="up", k=3), dict(grg=gc, direction="down", k=2),
+            dict(grg=ga, direction="up", k=4, by_individual=True, init_mode="xtx"),
+        ]
+        grids = [None, 7] if persistent else [None]
+        for grid in grids:
+            batch = rt.prepare_batched_sweep(entries, persistent=persistent, grid=grid)
+            assert batch.num_phases >= max(g.height for g in oracles)
+            inputs = []
+            for e, view, g in zip(entries, batch.inputs, (oa, oa, ob, ob, oc, oc, oa)):
+                cols = g.num_individuals if e.get("by_individual") else _cols(g, e["direction"])
+                x = rng.standard_normal((e["k"], cols))
+                view.copy_(torch.from_numpy(x))
+                inputs.append(x)
+            miss = rng.standard_normal((16, oa.num_mutations))
+            batch.miss_inputs[1].copy_(torch.from_numpy(miss))
+            batch()
+            batch.check()
+            want = [O.matmul(oa, inputs[0], "up"), O.matmul(oa, inputs[1], "down", miss=miss),
+                    O.matmul(ob, inputs[2], "up"), O.matmul(ob, inputs[3], "down"),
+                    O.matmul(oc, inputs[4], "up"), O.matmul(oc, inputs[5], "down"),
+                    O.matmul(oa, inputs[6], "up", by_individual=True, init="xtx")]
+            for out, w in zip(batch.outputs, want):
+                _check(out.cpu().numpy(), w)
+
+
+def test_batched_sweep_captured_and_replayed(art_a, art_b):
+    """A persistent batched up + down pair captured into one hipGraph: every replay
+    (with eager work and synchronizations between replays) equals the eager run
+    bit for bit, and +-1 inputs give the oracle's integers exactly."""
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    arts = [art_a, art_b]
+    oracles = [O.load_artifact(p) for p in arts]
+    with torch.cuda.device(0):
+        stream = torch.cuda.Stream()
+    layout = build_hip_layout(arts, requirements=full_requirements(max_k_up=8, max_k_down=8), stream=stream)
+    rng = np.random.default_rng(3860)
+    with HipRuntime(layout) as rt:
+        ga, gb = rt.grgs
+        up = rt.prepare_batched_sweep([dict(grg=ga, direction="up", k=8), dict(grg=gb, direction="up", k=3)])
+        dn = rt.prepare_batched_sweep([dict(grg=ga, direction="down", k=2), dict(grg=gb, direction="down", k=8)])
+        xs = [rng.choice(np.array([-1.0, 1.0]), size=(8, oracles[0].num_samples)),
+              rng.choice(np.array([-1.0, 1.0]), size=(3, oracles[1].num_samples))]
+        ys = [rng.choice(np.array([-1.0, 1.0]), size=(2, oracles[0].num_mutations)),
+              rng.choice(np.array([-1.0, 1.0]), size=(8, oracles[1].num_mutations))]
+        dev_x = [torch.from_numpy(x).to(rt.device) for x in xs]
+        dev_y = [torch.from_numpy(y).to(rt.device) for y in ys]
+
+        def run():
+            for v, x in zip(up.inputs, dev_x):
+                v.copy_(x)
+            for v, y in zip(dn.inputs, dev_y):
+                v.copy_(y)
+            up()
+            dn()
+
+        with torch.cuda.stream(stream):
+            run()
+        torch.cuda.synchronize()
+        eager = [o.cpu().clone() for o in (*up.outputs, *dn.outputs)]
+        for o, g, x in zip(up.outputs, oracles, xs):
+            np.testing.assert_array_equal(o.cpu().numpy(), O.matmul(g, x, "up"))
+        for o, g, y in zip(dn.outputs, oracles, ys):
+            np.testing.assert_array_equal(o.cpu().numpy(), O.matmul(g, y, "down"))
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            run()
+        for rep in range(3):
+            for o in (*up.outputs, *dn.outputs):
+                o.zero_()
+            torch.cuda.synchronize()
+            graph.replay()
+            torch.cuda.synchronize()
+            up.check()
+            got = [o.cpu() for o in (*up.outputs, *dn.outputs)]
+            assert all(torch.equal(a, b) for a, b in zip(got, eager)), f"replay {rep} differs from the eager run"
+
+
 def test_two_runtimes_on_separate_streams(art_a, art_b):
     import torch
 

@@ -212,7 +212,8 @@ def _declared_functions(header: Path) -> list[str]:
     return re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(grg_\w+)\s*\(", text, flags=re.M)
 
 
-@pytest.mark.parametrize("header,lib", [("grg_hip.h", "libgrg_hip.so"), ("grg_synth.h", "libgrg_synth.so")])
+@pytest.mark.parametrize("header,lib", [("grg_hip.h", "libgrg_hip.so"), ("grg_synth.h", "libgrg_synth.so"),
+                                        ("grg_host.h", "libgrg_host.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     path = REPO / "grg-spmv_amd" / "pygrgl_spmv" / "_lib" / lib
     if not path.exists():
@@ -228,7 +229,8 @@ def test_ctypes_struct_layout_matches_c_header(tmp_path):
     """sizeof / offsetof of GrgHipGatherArgs from the C compiler equal ctypes' layout."""
     from pygrgl_spmv.backends.hip import ffi
 
-    structs = {"GrgHipGatherArgs": ffi.GrgHipGatherArgs, "GrgHipSeg": ffi.GrgHipSeg, "GrgHipMap": ffi.GrgHipMap}
+    structs = {"GrgHipGatherArgs": ffi.GrgHipGatherArgs, "GrgHipSeg": ffi.GrgHipSeg, "GrgHipMap": ffi.GrgHipMap,
+               "GrgHipJob": ffi.GrgHipJob}
     exprs, want = [], []
     for cname, cls in structs.items():
         exprs.append(f"sizeof({cname})")
@@ -260,3 +262,36 @@ def test_hip_library_reports_errors_without_gpu():
     bad.dtype, bad.k, bad.cpl, bad.kl, bad.es, bad.row_end = ffi.F64, 3, 2, 1, 1, 1
     with pytest.raises(RuntimeError, match="multiple of cpl"):
         lib.gather(bad, 0)
+
+
+def _light_args(rows, k, kl=4, es=4, cpl=1):
+    from pygrgl_spmv.backends.hip import ffi
+
+    a = ffi.GrgHipGatherArgs()
+    a.dtype, a.k, a.cpl, a.kl, a.es, a.row_begin, a.row_end = ffi.F64, k, cpl, kl, es, 0, rows
+    a.out, a.ld_out, a.heavy_threshold = 256, k, -1
+    return a
+
+
+def test_job_table_layout_without_gpu():
+    """grg_hip_jobs_build: jobs laid end to end within each phase (light blocks of
+    4 waves x 64 / (kl * es) rows, times the column chunks), phase offsets, errors."""
+    from pygrgl_spmv.backends.hip import ffi
+
+    if not ffi.LIB_PATH.exists():
+        pytest.skip("libgrg_hip.so not built")
+    lib = ffi.HipLibrary()
+    args = [_light_args(100, 8, kl=4, es=4), _light_args(33, 16, kl=4, es=16, cpl=2), _light_args(5, 1, kl=1, es=64)]
+    jobs, off, blocks = lib.jobs_build(args, [0, 0, 2])
+    # job 0: 16 rows per block -> 7 blocks x ceil(8 / 4) = 2 chunks; job 1: 4 rows per block -> 9 x 2;
+    # job 2: 4 rows per block -> 2 x 1
+    assert [jobs[i].gx for i in range(3)] == [7, 9, 2]
+    assert [jobs[i].gy for i in range(3)] == [2, 2, 1]
+    assert [jobs[i].block_begin for i in range(3)] == [0, 14, 0]
+    assert list(off) == [0, 2, 2, 3] and list(blocks) == [32, 0, 2]
+    with pytest.raises(RuntimeError, match="non-decreasing"):
+        lib.jobs_build(args, [1, 0, 2])
+    tiled = _light_args(10, 4)
+    tiled.flags = ffi.FLAG_TILED
+    with pytest.raises(RuntimeError, match="job 0"):
+        lib.jobs_build([tiled], [0])
