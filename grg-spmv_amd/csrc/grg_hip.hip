@@ -204,6 +204,52 @@ __device__ __forceinline__ void gather_rows(const int32_t* __restrict__ idx, con
   }
 }
 
+// gather_rows over merged entry keys: each entry names its segment in the top
+// two bits (3 = absent), so one round serves all segments of the R rows.
+template <typename T, int CPL, int U, int R, typename Acc>
+__device__ __forceinline__ void gather_rows_keyed(const uint32_t* __restrict__ ekey, const T* src0, const T* src1,
+                                                  const T* src2, int64_t ld0, int64_t ld1, int64_t ld2, int32_t (&j)[R],
+                                                  const int32_t (&end)[R], int32_t stride, Acc (&acc)[R][CPL]) {
+  while (true) {
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) any |= j[t] < end[t];
+    if (!any) break;
+    uint32_t kk[R][U];
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t jt = j[t] + u * stride;
+        kk[t][u] = jt < end[t] ? ekey[jt] : 0xffffffffu;
+      }
+    T v[R][U][CPL];
+#pragma unroll
+    for (int t = 0; t < R; ++t)
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t sg = kk[t][u] >> 30;
+        if (sg != 3u) {
+          const int64_t r = (int64_t)(kk[t][u] & 0x3fffffffu);
+          const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
+          const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
+          load_raw<T, CPL>(base + r * ld, v[t][u]);
+        } else {
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) v[t][u][c] = T(0);
+        }
+      }
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[t][c].add(v[t][u][c]);
+      j[t] += U * stride;
+    }
+  }
+}
+
 // Row-wise seed / epilogue terms (added once per row by one lane).
 template <typename T, int CPL, typename Acc>
 __device__ __forceinline__ void add_row_terms(const GrgHipGatherArgs& a, int64_t row, int col, Acc (&acc)[CPL]) {
@@ -383,7 +429,31 @@ __device__ __forceinline__ void gather_block(const GrgHipGatherArgs& a, int64_t 
 #pragma unroll
       for (int c = 0; c < CPL; ++c) racc[t][c].zero();
     }
-    if (any_live) {
+    if (any_live && (a.flags & GRG_HIP_FLAG_MERGED)) {
+      // one pointer pair per row over the merged entries of all segments
+      int32_t pb[R + 1];
+#pragma unroll
+      for (int t = 0; t <= R; ++t) pb[t] = a.eptr[row0 + t < a.row_end ? row0 + t : a.row_end];
+      int32_t j[R], end[R];
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        live[t] = row0 + t < a.row_end && !(a.heavy_threshold >= 0 && pb[t + 1] - pb[t] > a.heavy_threshold);
+        j[t] = pb[t] + e;
+        end[t] = live[t] ? pb[t + 1] : 0;
+      }
+      const T* src0 = static_cast<const T*>(a.seg[0].src) + col;
+      const T* src1 = (a.nseg > 1 ? static_cast<const T*>(a.seg[1].src) : static_cast<const T*>(a.seg[0].src)) + col;
+      const T* src2 = (a.nseg > 2 ? static_cast<const T*>(a.seg[2].src) : static_cast<const T*>(a.seg[0].src)) + col;
+      const int64_t ld0 = a.seg[0].ld;
+      const int64_t ld1 = a.nseg > 1 ? a.seg[1].ld : ld0;
+      const int64_t ld2 = a.nseg > 2 ? a.seg[2].ld : ld0;
+      gather_rows_keyed<T, CPL, U, R>(a.ekey, src0, src1, src2, ld0, ld1, ld2, j, end, ES, racc);
+      if (e == 0) {
+#pragma unroll
+        for (int t = 0; t < R; ++t)
+          if (live[t]) add_row_terms<T, CPL>(a, row0 + t, col, racc[t]);
+      }
+    } else if (any_live) {
       // row pointers of the R rows in every segment (rows past row_end read the level's end)
       int32_t pb[GRG_HIP_MAX_SEGS][R + 1];
       int64_t deg[R];
@@ -744,7 +814,10 @@ int validate(const GrgHipGatherArgs& a) {
     const bool ok = (r == 1 && (u == 4 || u == 8 || u == 16)) || (r == 2 && (u == 4 || u == 8)) || (r == 4 && u == 4);
     if (!ok) return fail("(unroll, rows_per_group) must be one of (4|8|16, 1), (4|8, 2), (4, 4)");
   }
-  if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_TILED)) return fail("unknown flags");
+  if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_MERGED))
+    return fail("unknown flags");
+  if ((a.flags & GRG_HIP_FLAG_MERGED) && (!a.eptr || !a.ekey || a.nseg < 1))
+    return fail("merged-entry launch needs eptr, ekey and at least one segment");
   if ((a.flags & GRG_HIP_FLAG_TILED) && (a.flags & GRG_HIP_FLAG_PARTIALS)) return fail("a launch is either tiled or a partials launch");
   if (a.k % a.cpl) return fail("k must be a multiple of cpl");
   if (!a.out) return fail("out is null");

@@ -105,6 +105,7 @@ void build_tiles(int64_t R, int32_t L, const int64_t* off, int32_t nseg, const i
   if (R < 0 || L < 0 || nseg < 1 || nseg > 3) throw std::invalid_argument("bad sizes");
   if (max_staged < 1 || max_staged > 65535 || max_rows < 1 || max_entries < 1) throw std::invalid_argument("bad tile limits");
   if (off[0] != 0 || off[L] != R) throw std::invalid_argument("level offsets do not cover the rows");
+  if (eptr[0] != 0) throw std::invalid_argument("eptr must start at 0");
   // stamp[s][src] = id of the last tile that staged src, pos[s][src] = its staged position there
   std::vector<std::vector<int32_t>> stamp(nseg);
   std::vector<std::vector<uint16_t>> pos(nseg);
@@ -226,6 +227,36 @@ int grg_host_build_tiles(int64_t num_rows, int32_t num_levels, const int64_t* le
     g_last_error = e.what();
     return 2;
   }
+}
+
+int grg_host_merged_keys(int64_t num_rows, int32_t nseg, const int32_t* const* seg_ptr, const int32_t* const* seg_idx,
+                         const int32_t* eptr, uint32_t* ekey) {
+  if (!seg_ptr || !seg_idx || !eptr || !ekey || num_rows < 0 || nseg < 1 || nseg > 3) {
+    g_last_error = "bad arguments";
+    return 1;
+  }
+  if (eptr[0] != 0) {
+    g_last_error = "eptr must start at 0";
+    return 2;
+  }
+  for (int64_t r = 0; r < num_rows; ++r) {
+    int64_t e = eptr[r];
+    for (int32_t s = 0; s < nseg; ++s) {
+      for (int32_t j = seg_ptr[s][r]; j < seg_ptr[s][r + 1]; ++j, ++e) {
+        const int32_t c = seg_idx[s][j];
+        if (c < 0 || c >= (1 << 30)) {
+          g_last_error = "source row outside [0, 2^30)";
+          return 2;
+        }
+        ekey[e] = ((uint32_t)s << 30) | (uint32_t)c;
+      }
+    }
+    if (e != eptr[r + 1]) {
+      g_last_error = "eptr does not match the segment degrees";
+      return 2;
+    }
+  }
+  return 0;
 }
 
 const char* grg_host_last_error(void) { return g_last_error.c_str(); }

@@ -31,6 +31,8 @@ def _lib():
                                              c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                              c_void_p, c_void_p]
         lib.grg_host_build_tiles.restype = ctypes.c_int
+        lib.grg_host_merged_keys.argtypes = [c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]
+        lib.grg_host_merged_keys.restype = ctypes.c_int
         lib.grg_host_last_error.argtypes = []
         lib.grg_host_last_error.restype = ctypes.c_char_p
         _LIB = lib
@@ -132,4 +134,22 @@ def build_tiles(level_offsets: np.ndarray, segments, heavy_threshold, *, max_sta
                  level_max_rows=level_max[L:2 * L].copy(), level_max_entries=level_max[2 * L:3 * L].copy())
 
 
-__all__ = ["LIB_PATH", "LIDX_PAD", "Tiles", "build_tiles", "locality_order"]
+def merged_keys(eptr: np.ndarray, segments) -> np.ndarray:
+    """uint32 (segment << 30) | source row of every entry, rows' entries over all
+    segments in segment order at ``eptr`` (``grg_host_merged_keys``)."""
+    lib = _lib()
+    eptr = np.ascontiguousarray(eptr, dtype=np.int32)
+    R = eptr.size - 1
+    ptrs = [np.ascontiguousarray(p, dtype=np.int32) for p, _, _ in segments]
+    idxs = [np.ascontiguousarray(i, dtype=np.int32) for _, i, _ in segments]
+    nseg = len(segments)
+    ekey = np.empty(max(int(eptr[-1]), 1), dtype=np.uint32)
+    pp = (c_void_p * nseg)(*[p.ctypes.data for p in ptrs])
+    ip = (c_void_p * nseg)(*[i.ctypes.data for i in idxs])
+    rc = lib.grg_host_merged_keys(R, nseg, ctypes.cast(pp, c_void_p), ctypes.cast(ip, c_void_p), eptr.ctypes.data, ekey.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"grg_host_merged_keys failed: status {rc}: {lib.grg_host_last_error().decode()}")
+    return ekey
+
+
+__all__ = ["LIB_PATH", "LIDX_PAD", "Tiles", "build_tiles", "locality_order", "merged_keys"]

@@ -257,19 +257,20 @@ def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequiremen
 
     Per sweep configuration (up; down; down with missing-data input): a uint16
     staged index per gather entry (plus 16 bytes of padding), at most one uint32
-    staged source per entry,
+    staged source per entry, a uint32 merged entry key per entry (configurations
+    of more than one segment),
     int32 merged row pointers, and int32 tile row and staged-source offsets (at
     most one tile per row plus one per level).
     """
     N, L, E = scan.num_nodes, scan.num_levels, scan.num_edges
-    entries = []
+    entries = []  # (entries, merged keys kept: more than one segment)
     if pair.plan_up is not None:
-        entries.append(E)
+        entries.append((E, False))
     if pair.plan_down is not None:
-        entries.append(E + scan.selector_mut_nnz)
+        entries.append((E + scan.selector_mut_nnz, True))
         if req.need_down_miss_input:
-            entries.append(E + scan.selector_mut_nnz + scan.selector_miss_nnz)
-    return int(sum(6 * e + 16 + 4 * (N + 1) + 8 * (N + L + 1) for e in entries))
+            entries.append((E + scan.selector_mut_nnz + scan.selector_miss_nnz, True))
+    return int(sum(6 * e + (4 * e if keyed else 0) + 16 + 4 * (N + 1) + 8 * (N + L + 1) for e, keyed in entries))
 
 
 def plan_hip_layout(

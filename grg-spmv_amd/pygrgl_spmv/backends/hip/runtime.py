@@ -673,7 +673,7 @@ class HipRuntime:
         return ptr_all.to(torch.int32), idx, deg
 
     def _build_artifact(self, ai: int, al, st) -> _HipArtifact:
-        from pygrgl_spmv.backends.hip.host import build_tiles, locality_order
+        from pygrgl_spmv.backends.hip.host import build_tiles, locality_order, merged_keys
 
         torch = _torch()
         lay = self.layout
@@ -844,6 +844,8 @@ class HipRuntime:
                     D[f"{name}_src_list"] = self._to_dev(t.src_list.view(np.int32))
                     D[f"{name}_eptr"] = self._to_dev(t.eptr)
                     D[f"{name}_lidx"] = self._to_dev(t.lidx.view(np.int16))
+                    if len(segs_h) > 1:
+                        D[f"{name}_ekey"] = self._to_dev(merged_keys(t.eptr, segs_h).view(np.int32))
                     art.tiles[name] = (t.level_tile_off.copy(), t.level_max_staged.copy(), t.level_max_rows.copy(),
                                        t.level_max_entries.copy())
                     del t
@@ -873,7 +875,8 @@ class HipRuntime:
         return {"tile_row": D[f"{cfg}_tile_row"].data_ptr() + 4 * t0, "tile_src": D[f"{cfg}_tile_src"].data_ptr() + 4 * t0,
                 "src_list": D[f"{cfg}_src_list"].data_ptr(), "eptr": D[f"{cfg}_eptr"].data_ptr(),
                 "lidx": D[f"{cfg}_lidx"].data_ptr(), "num_tiles": t1 - t0, "tile_lds_rows": int(level_max_staged[level]),
-                "tile_max_rows": int(level_max_rows[level]), "tile_max_entries": int(level_max_entries[level])}
+                "tile_max_rows": int(level_max_rows[level]), "tile_max_entries": int(level_max_entries[level]),
+                "ekey": D[f"{cfg}_ekey"].data_ptr() if f"{cfg}_ekey" in D else 0}
 
     def _args(self, *, rows, k, out, ld_out, segs=(), maps=(), row_scale=None, col_vec=None, accumulate=False,
               sched: _LevelSchedule | None = None, sched_base: int = 0, mean_degree: float = 1.0,
@@ -906,6 +909,8 @@ class HipRuntime:
                 setattr(a, name, int(value))
         for name, value in self.fixed_shape.items():
             setattr(a, name, int(value))
+        if (a.flags & ffi.FLAG_MERGED) and not a.ekey:
+            a.flags &= ~ffi.FLAG_MERGED
         if a.flags & ffi.FLAG_TILED:
             if not tiles:
                 a.flags &= ~ffi.FLAG_TILED
@@ -971,7 +976,14 @@ class HipRuntime:
                         continue
                     time_shape(cpl, kl, es, unroll, rows, 0)
                 _, cpl, kl, es, unroll, rows, _ = min(timings)
-                time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_XCD_REMAP)
+                if a.ekey:
+                    # the merged-entry light path on the best shape and its neighbours in es
+                    for es_m in sorted({es, max(1, es // 2), min(64 // kl, es * 2)}):
+                        time_shape(cpl, kl, es_m, unroll, rows, ffi.FLAG_MERGED)
+                    _, cpl, kl, es, unroll, rows, fl = min(timings)
+                    time_shape(cpl, kl, es, unroll, rows, (fl & ffi.FLAG_MERGED) | ffi.FLAG_XCD_REMAP)
+                else:
+                    time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_XCD_REMAP)
                 if a.tile_row:
                     # the LDS-staged form of the same launch: column chunks of at most
                     # TILE_CHUNK_BYTES, every lane-group width

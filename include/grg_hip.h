@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 8
+#define GRG_HIP_ABI_VERSION 9
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -57,6 +57,10 @@ enum GrgHipDtype {
 #define GRG_HIP_FLAG_PARTIALS 4
 /* Tiled launch: see the tile fields of GrgHipGatherArgs. */
 #define GRG_HIP_FLAG_TILED 8
+/* Merged-entry light path: the per-row lane groups read a row's entries over
+ * all segments from eptr / ekey (one pointer pair and one round of gathers per
+ * row) instead of segment by segment; heavy rows still use the segments. */
+#define GRG_HIP_FLAG_MERGED 16
 /* Bytes of one staged row (kl * cpl columns) in a tiled launch, at most. */
 #define GRG_HIP_TILE_CHUNK_BYTES 128
 /* Staged rows of one tile, at most (uint16 LDS indices). */
@@ -149,6 +153,10 @@ typedef struct GrgHipGatherArgs {
   int64_t tile_lds_rows;
   int64_t tile_max_rows;
   int64_t tile_max_entries;
+  /* Merged entry keys (GRG_HIP_FLAG_MERGED; no reference counterpart): row r's
+   * entries over all segments, in segment order, are ekey[eptr[r] ..
+   * eptr[r + 1]), each (segment << 30) | source row. */
+  const uint32_t* ekey;
 } GrgHipGatherArgs;
 
 /* One row-gather launch (see above). Replaces one level of the reference's

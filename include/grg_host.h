@@ -67,6 +67,13 @@ int grg_host_build_tiles(int64_t num_rows, int32_t num_levels, const int64_t* le
                          int32_t max_entries, int32_t* tile_row, int32_t* tile_src, uint32_t* src_list, uint16_t* lidx,
                          int64_t* level_tile_off, int32_t* level_max);
 
+/* Merged entry keys for the merged-entry launches of libgrg_hip.so: row r's
+ * entries over all nseg segments, in segment order, as (segment << 30) |
+ * source row, written at ekey[eptr[r] ..] (eptr as for grg_host_build_tiles,
+ * starting at 0, with ekey holding eptr[num_rows] keys; source rows below 2^30). No reference counterpart. Returns 0 on success. */
+int grg_host_merged_keys(int64_t num_rows, int32_t nseg, const int32_t* const* seg_ptr, const int32_t* const* seg_idx,
+                         const int32_t* eptr, uint32_t* ekey);
+
 const char* grg_host_last_error(void);
 
 #ifdef __cplusplus

@@ -116,14 +116,17 @@ def test_untuned_launch_shapes(art_a, oracle_a, k):
 
 @pytest.mark.parametrize("unroll,rows", [(4, 1), (8, 1), (16, 1), (4, 2), (8, 2), (4, 4)])
 @pytest.mark.parametrize("es", [1, 4])
-def test_every_kernel_variant(art_a, oracle_a, unroll, rows, es):
+@pytest.mark.parametrize("merged", [False, True])
+def test_every_kernel_variant(art_a, oracle_a, unroll, rows, es, merged):
     """Each (gathers in flight, rows per lane group) variant of the light path, with the
-    XCD remap, against the oracle in every sweep configuration."""
+    XCD remap, per segment or over merged entry keys, against the oracle in every
+    sweep configuration."""
     from pygrgl_spmv.backends.hip import HipRuntime
     from pygrgl_spmv.backends.hip import ffi
 
     layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=8, max_k_down=8))
-    shape = {"unroll": unroll, "rows_per_group": rows, "es": es, "flags": ffi.FLAG_XCD_REMAP}
+    shape = {"unroll": unroll, "rows_per_group": rows, "es": es,
+             "flags": ffi.FLAG_XCD_REMAP | (ffi.FLAG_MERGED if merged else 0)}
     with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16) as rt:
         (grg,) = rt.grgs
         rng = np.random.default_rng(3350 + 10 * unroll + rows + es)

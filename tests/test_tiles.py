@@ -109,3 +109,21 @@ def test_builder_rejects_bad_input():
         build_tiles(off, [(p, i, 5)], np.array([-1]))
     with pytest.raises(ValueError, match="cover every row"):
         build_tiles(off, [(p[:3], i, 10)], np.array([-1]))
+
+
+def test_merged_keys_follow_segment_order():
+    from pygrgl_spmv.backends.hip.host import merged_keys
+
+    rng = np.random.default_rng(11)
+    R = 50
+    segs = [(*_random_csr(rng, R, 40, 5), 40), (*_random_csr(rng, R, 7, 2), 7), (*_random_csr(rng, R, 9, 3), 9)]
+    deg = sum(np.diff(p.astype(np.int64)) for p, _, _ in segs)
+    eptr = np.zeros(R + 1, dtype=np.int64)
+    np.cumsum(deg, out=eptr[1:])
+    keys = merged_keys(eptr.astype(np.int32), segs)
+    for r in range(R):
+        want = np.concatenate([(np.uint32(s) << np.uint32(30)) | i[p[r]:p[r + 1]].astype(np.uint32)
+                               for s, (p, i, _) in enumerate(segs)])
+        np.testing.assert_array_equal(keys[eptr[r]:eptr[r + 1]], want)
+    with pytest.raises(RuntimeError, match="eptr"):
+        merged_keys((eptr + 1).astype(np.int32), segs)
