@@ -133,4 +133,25 @@ class MklSweep:
             self.lib.MKL_Set_Num_Threads_Local(prev)
 
 
-__all__ = ["MklSweep"]
+def mkl_matmul(g, sweep: MklSweep, x: np.ndarray, direction: str, *, miss: np.ndarray | None = None) -> np.ndarray:
+    """``BoundGRG.matmul`` without init modes or by-individual columns, with the level
+    sweep done by ``sweep`` (the reference's MklRuntime arithmetic,
+    ``mkl/backend.py:559-645``) and the selectors by SciPy: up seeds the sample rows
+    with ``x`` (k, S) and returns ``sel_mut @ V`` (k, M); down seeds with
+    ``sel_mut^T y`` (+ ``sel_miss^T miss``) and returns the sample rows (k, S)."""
+    xi = np.ascontiguousarray(np.asarray(x, dtype=np.float64).T)
+    k = xi.shape[1]
+    if direction == "up":
+        v = np.zeros((g.num_nodes, k))
+        v[: g.num_samples] = xi
+        sweep.sweep(v, "up")
+        return np.asarray(g.sel_mut @ v).T.copy()
+    seed = g.sel_mut.T @ xi
+    if miss is not None:
+        seed = seed + g.sel_miss.T @ np.asarray(miss, dtype=np.float64).T
+    v = np.ascontiguousarray(seed, dtype=np.float64)
+    sweep.sweep(v, "down")
+    return v[: g.num_samples].T.copy()
+
+
+__all__ = ["MklSweep", "mkl_matmul"]
