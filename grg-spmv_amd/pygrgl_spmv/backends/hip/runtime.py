@@ -68,9 +68,9 @@ def _pow2_floor(v: float) -> int:
 
 
 KERNEL_UNROLL = 8  # gathers in flight per lane and round in the kernel
-TILE_MAX_STAGED = 256   # staged source rows of one tile (LDS rows)
-TILE_MAX_ROWS = 512     # rows of one tile
-TILE_MAX_ENTRIES = 4096  # gather entries of one tile
+TILE_MAX_STAGED = 128   # staged source rows of one tile (LDS rows)
+TILE_MAX_ROWS = 256     # rows of one tile
+TILE_MAX_ENTRIES = 2048  # gather entries of one tile
 
 
 def column_shapes(dtype_code: int, k: int, lds) -> list[tuple[int, int]]:
