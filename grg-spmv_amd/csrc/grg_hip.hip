@@ -46,6 +46,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 
@@ -517,6 +518,147 @@ __global__ __launch_bounds__(kBlock) void row_gather_kernel(const GrgHipGatherAr
   gather_block<T, CPL, U, R>(a, (int64_t)blockIdx.x, (int)blockIdx.y, n_light_blocks, g_shift, kl_shift, partial);
 }
 
+// Streamed light path (GRG_HIP_FLAG_STREAM): each wave loops over batches of
+// 64 / G rows (one per lane group), batch b, b + W, ... for the W waves of the
+// grid. A row costs three dependent loads (row pointers -> entry keys -> node
+// rows); the loop runs them as a three-stage pipeline, issuing in one
+// iteration the pointers of batch i + 2, the keys of batch i + 1 and the first
+// U gathers of batch i, so a row waits for one round trip instead of three.
+// Entries beyond the first U per lane (long rows) are gathered in place. Keys
+// are merged entry keys (GRG_HIP_FLAG_MERGED) or, with one segment, its
+// indices. Heavy rows follow, one workgroup each, over the same grid.
+// Keys of the first U entries (per lane) of a row [p0, p1) of a streamed
+// launch; absent entries (and rows that are not live) read as 0xffffffff.
+template <int U>
+__device__ __forceinline__ void stream_keys(const uint32_t* __restrict__ keys, int32_t p0, int32_t p1, bool live, int e,
+                                            int ES, uint32_t (&kk)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int32_t j = p0 + e + u * ES;
+    kk[u] = live && j < p1 ? keys[j] : 0xffffffffu;
+  }
+}
+
+// Gathers of U keyed entries (segment in the top two bits, 3 = absent).
+template <typename T, int CPL, int U>
+__device__ __forceinline__ void stream_gather(const uint32_t (&kk)[U], const T* src0, const T* src1, const T* src2,
+                                              int64_t ld0, int64_t ld1, int64_t ld2, T (&v)[U][CPL]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint32_t sg = kk[u] >> 30;
+    if (sg != 3u) {
+      const int64_t r = (int64_t)(kk[u] & 0x3fffffffu);
+      const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
+      const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
+      load_raw<T, CPL>(base + r * ld, v[u]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) v[u][c] = T(0);
+    }
+  }
+}
+
+template <typename T, int CPL, int U>
+__global__ __launch_bounds__(kBlock) void row_stream_kernel(const GrgHipGatherArgs a, int g_shift, int kl_shift) {
+  using Acc = typename SumOf<T>::type;
+  __shared__ Acc partial[kWavesPerBlock][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int KL = 1 << kl_shift;
+  const int G = 1 << g_shift;
+  const int lg = lane & (G - 1);
+  const int e = lg >> kl_shift;
+  const int ES = G >> kl_shift;
+  const int col = ((int)blockIdx.y * KL + (lg & (KL - 1))) * CPL;
+  const bool merged = (a.flags & GRG_HIP_FLAG_MERGED) != 0;
+  const int32_t* ptr = merged ? a.eptr : a.seg[0].ptr - a.seg[0].ptr_base;
+  const uint32_t* keys = merged ? a.ekey : reinterpret_cast<const uint32_t*>(a.seg[0].idx);
+  const T* src0 = static_cast<const T*>(a.seg[0].src) + col;
+  const T* src1 = (a.nseg > 1 ? static_cast<const T*>(a.seg[1].src) : static_cast<const T*>(a.seg[0].src)) + col;
+  const T* src2 = (a.nseg > 2 ? static_cast<const T*>(a.seg[2].src) : static_cast<const T*>(a.seg[0].src)) + col;
+  const int64_t ld0 = a.seg[0].ld;
+  const int64_t ld1 = a.nseg > 1 ? a.seg[1].ld : ld0;
+  const int64_t ld2 = a.nseg > 2 ? a.seg[2].ld : ld0;
+  const int64_t rows_per_wave = 64 >> g_shift;
+  const int64_t n_batches = (a.row_end - a.row_begin + rows_per_wave - 1) / rows_per_wave;
+  const int64_t n_waves = (int64_t)gridDim.x * kWavesPerBlock;
+  const int64_t my_row = lane >> g_shift;
+  const bool col_live = col < a.k;
+  const int64_t thr = a.heavy_threshold;
+
+  // batch b's row for this lane group: r, its pointers [p0, p1) (p1 < p0: no row)
+  int64_t b = (int64_t)blockIdx.x * kWavesPerBlock + wave;
+  int64_t rc = a.row_begin + b * rows_per_wave + my_row;
+  int64_t rn = rc + n_waves * rows_per_wave;
+  int32_t c0 = 0, c1 = -1, n0 = 0, n1 = -1;
+  if (b < n_batches && rc < a.row_end) {
+    c0 = ptr[rc];
+    c1 = ptr[rc + 1];
+  }
+  if (b + n_waves < n_batches && rn < a.row_end) {
+    n0 = ptr[rn];
+    n1 = ptr[rn + 1];
+  }
+  bool live_c = col_live && c1 >= c0 && !(thr >= 0 && c1 - c0 > thr);
+  uint32_t kc[U];
+  stream_keys<U>(keys, c0, c1, live_c, e, ES, kc);
+  while (b < n_batches) {
+    // pointers of batch b + 2W, keys of batch b + W, gathers of batch b: independent loads
+    const int64_t rf = rn + n_waves * rows_per_wave;
+    int32_t f0 = 0, f1 = -1;
+    if (b + 2 * n_waves < n_batches && rf < a.row_end) {
+      f0 = ptr[rf];
+      f1 = ptr[rf + 1];
+    }
+    const bool live_n = col_live && n1 >= n0 && !(thr >= 0 && n1 - n0 > thr);
+    uint32_t kn[U];
+    stream_keys<U>(keys, n0, n1, live_n, e, ES, kn);
+    T v[U][CPL];
+    stream_gather<T, CPL, U>(kc, src0, src1, src2, ld0, ld1, ld2, v);
+    Acc acc[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) acc[c].zero();
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) acc[c].add(v[u][c]);
+    if (live_c) {
+      // entries past the first U per lane: gathered in place
+      for (int32_t j = c0 + e + U * ES; j < c1; j += U * ES) {
+        uint32_t kx[U];
+        stream_keys<U>(keys, j - e, c1, true, e, ES, kx);
+        T vx[U][CPL];
+        stream_gather<T, CPL, U>(kx, src0, src1, src2, ld0, ld1, ld2, vx);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) acc[c].add(vx[u][c]);
+      }
+      if (e == 0) add_row_terms<T, CPL>(a, rc, col, acc);
+    }
+    for (int off = KL; off < G; off <<= 1) {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) acc[c].merge_xor(off);
+    }
+    if (live_c && e == 0) store_row<T, CPL>(a, rc, col, acc);
+    b += n_waves;
+    rc = rn;
+    rn = rf;
+    c0 = n0;
+    c1 = n1;
+    n0 = f0;
+    n1 = f1;
+    live_c = live_n;
+#pragma unroll
+    for (int u = 0; u < U; ++u) kc[u] = kn[u];
+  }
+  // heavy rows: one workgroup each, strided over the grid
+  for (int64_t h = blockIdx.x; h < a.num_heavy; h += gridDim.x) {
+    workgroup_row<T, CPL, U>(a, a.heavy_rows[h], h, (int)blockIdx.y, kl_shift, partial);
+    __syncthreads();
+  }
+}
+
 // Padding (elements) after each staged row of a tile: at least 16 bytes and one
 // lane vector, so rows start on a vector boundary and consecutive rows start in
 // different LDS banks.
@@ -814,8 +956,16 @@ int validate(const GrgHipGatherArgs& a) {
     const bool ok = (r == 1 && (u == 4 || u == 8 || u == 16)) || (r == 2 && (u == 4 || u == 8)) || (r == 4 && u == 4);
     if (!ok) return fail("(unroll, rows_per_group) must be one of (4|8|16, 1), (4|8, 2), (4, 4)");
   }
-  if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_MERGED))
+  if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_MERGED |
+                  GRG_HIP_FLAG_STREAM))
     return fail("unknown flags");
+  if (a.flags & GRG_HIP_FLAG_STREAM) {
+    if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PARTIALS)) return fail("a streamed launch is neither tiled nor a partials launch");
+    if (!(a.flags & GRG_HIP_FLAG_MERGED) && a.nseg != 1) return fail("a streamed launch needs merged entries or one segment");
+    if (a.nseg < 1) return fail("a streamed launch needs at least one segment");
+    if (a.unroll != 0 && a.unroll != 4 && a.unroll != 8) return fail("a streamed launch runs 4 or 8 gathers in flight");
+    if (a.stream_blocks < 0) return fail("stream_blocks < 0");
+  }
   if ((a.flags & GRG_HIP_FLAG_MERGED) && (!a.eptr || !a.ekey || a.nseg < 1))
     return fail("merged-entry launch needs eptr, ekey and at least one segment");
   if ((a.flags & GRG_HIP_FLAG_TILED) && (a.flags & GRG_HIP_FLAG_PARTIALS)) return fail("a launch is either tiled or a partials launch");
@@ -887,11 +1037,58 @@ int launch_tiled_typed(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl
   return 0;
 }
 
+template <typename T, int CPL>
+int launch_stream_typed(const GrgHipGatherArgs& a, int64_t want_blocks, int gy, int g_shift, int kl_shift, hipStream_t s) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t err = hipStreamGetDevice(s, &dev);
+  if (err == hipSuccess) err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int u = a.unroll ? a.unroll : 8;
+  const void* fn = u == 4 ? reinterpret_cast<const void*>(&row_stream_kernel<T, CPL, 4>)
+                          : reinterpret_cast<const void*>(&row_stream_kernel<T, CPL, 8>);
+  if (err == hipSuccess) err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0);
+  if (err != hipSuccess) return fail(std::string("occupancy query: ") + hipGetErrorString(err), 2);
+  // resident workgroups (over the column chunks) bound the grid; each wave then loops
+  int64_t gx = (int64_t)cus * per_cu / (gy > 0 ? gy : 1);
+  if (a.stream_blocks > 0 && a.stream_blocks < gx) gx = a.stream_blocks;
+  if (gx > want_blocks) gx = want_blocks;
+  if (gx < 1) gx = 1;
+  dim3 grid((unsigned)gx, (unsigned)gy);
+  if (u == 4) hipLaunchKernelGGL((row_stream_kernel<T, CPL, 4>), grid, dim3(kBlock), 0, s, a, g_shift, kl_shift);
+  else hipLaunchKernelGGL((row_stream_kernel<T, CPL, 8>), grid, dim3(kBlock), 0, s, a, g_shift, kl_shift);
+  return 0;
+}
+
 int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
   if (int rc = validate(a)) return rc;
   const int G = a.kl * a.es;
   const int g_shift = ilog2(G), kl_shift = ilog2(a.kl);
   int64_t n_light = 0, gx = 0;
+  if (a.flags & GRG_HIP_FLAG_STREAM) {
+    const int64_t rows_per_block = (int64_t)kWavesPerBlock * (64 / G);
+    const int64_t want = std::max<int64_t>((a.row_end - a.row_begin + rows_per_block - 1) / rows_per_block, a.num_heavy);
+    if (want == 0) return 0;
+    const int chunk = a.kl * a.cpl;
+    const int gy = (a.k + chunk - 1) / chunk;
+    int rc = 0;
+    if (a.dtype == GRG_HIP_F64) {
+      switch (a.cpl) {
+        case 1: rc = launch_stream_typed<double, 1>(a, want, gy, g_shift, kl_shift, stream); break;
+        case 2: rc = launch_stream_typed<double, 2>(a, want, gy, g_shift, kl_shift, stream); break;
+        default: rc = launch_stream_typed<double, 4>(a, want, gy, g_shift, kl_shift, stream); break;
+      }
+    } else {
+      switch (a.cpl) {
+        case 1: rc = launch_stream_typed<float, 1>(a, want, gy, g_shift, kl_shift, stream); break;
+        case 2: rc = launch_stream_typed<float, 2>(a, want, gy, g_shift, kl_shift, stream); break;
+        case 8: rc = launch_stream_typed<float, 8>(a, want, gy, g_shift, kl_shift, stream); break;
+        default: rc = launch_stream_typed<float, 4>(a, want, gy, g_shift, kl_shift, stream); break;
+      }
+    }
+    if (rc) return rc;
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return fail(std::string("kernel launch failed: ") + hipGetErrorString(err), 2);
+    return 0;
+  }
   if (a.flags & GRG_HIP_FLAG_TILED) {
     gx = a.num_tiles + a.num_heavy;
     if (gx == 0) return 0;

@@ -13,13 +13,14 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 9
+ABI_VERSION = 10
 F64 = 0
 F32C = 1
 FLAG_XCD_REMAP = 2
 FLAG_PARTIALS = 4
 FLAG_TILED = 8
 FLAG_MERGED = 16
+FLAG_STREAM = 32
 TILE_CHUNK_BYTES = 128
 TILE_MAX_STAGED = 1024
 MAX_SEGS = 3
@@ -77,6 +78,7 @@ class GrgHipGatherArgs(ctypes.Structure):
         ("tile_max_rows", c_int64),
         ("tile_max_entries", c_int64),
         ("ekey", c_void_p),
+        ("stream_blocks", c_int64),
     ]
 
 

@@ -467,7 +467,7 @@ class HipRuntime:
         self.device = torch.device("cuda", int(layout.device))
         self.stream_ptr = int(layout.stream_ptr)
         self.autotune = bool(autotune) and not fixed_shape
-        allowed = {"unroll", "rows_per_group", "es", "flags", "cpl", "kl"}
+        allowed = {"unroll", "rows_per_group", "es", "flags", "cpl", "kl", "stream_blocks"}
         if fixed_shape and set(fixed_shape) - allowed:
             raise ValueError(f"fixed_shape keys must be among {sorted(allowed)}")
         self.fixed_shape = dict(fixed_shape or {})
@@ -911,6 +911,15 @@ class HipRuntime:
             setattr(a, name, int(value))
         if (a.flags & ffi.FLAG_MERGED) and not a.ekey:
             a.flags &= ~ffi.FLAG_MERGED
+        if a.flags & ffi.FLAG_STREAM:
+            # the streamed path reads merged keys, or the indices of a single segment
+            if a.nseg > 1 and a.ekey:
+                a.flags |= ffi.FLAG_MERGED
+            if a.nseg < 1 or (a.nseg > 1 and not a.ekey) or (a.flags & (ffi.FLAG_TILED | ffi.FLAG_PARTIALS)):
+                a.flags &= ~ffi.FLAG_STREAM
+            else:
+                a.unroll = 4 if a.unroll == 4 else 8
+                a.rows_per_group = 1
         if a.flags & ffi.FLAG_TILED:
             if not tiles:
                 a.flags &= ~ffi.FLAG_TILED
@@ -984,6 +993,13 @@ class HipRuntime:
                     time_shape(cpl, kl, es, unroll, rows, (fl & ffi.FLAG_MERGED) | ffi.FLAG_XCD_REMAP)
                 else:
                     time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_XCD_REMAP)
+                if a.nseg == 1 or a.ekey:
+                    # the streamed (software-pipelined) light path on the best column shape
+                    _, cpl, kl, es, unroll, rows, fl = min(timings)
+                    base = ffi.FLAG_STREAM | (ffi.FLAG_MERGED if a.nseg > 1 else 0)
+                    for es_s in sorted({es, max(1, es // 2), min(64 // kl, es * 2)}):
+                        for u in (4, 8):
+                            time_shape(cpl, kl, es_s, u, 1, base)
                 if a.tile_row:
                     # the LDS-staged form of the same launch: column chunks of at most
                     # TILE_CHUNK_BYTES, every lane-group width

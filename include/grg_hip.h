@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 9
+#define GRG_HIP_ABI_VERSION 10
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -61,6 +61,12 @@ enum GrgHipDtype {
  * all segments from eptr / ekey (one pointer pair and one round of gathers per
  * row) instead of segment by segment; heavy rows still use the segments. */
 #define GRG_HIP_FLAG_MERGED 16
+/* Streamed light path: a grid of at most the resident workgroups (or
+ * stream_blocks, when > 0) whose waves loop over the row batches, with the row
+ * pointers, entry keys and gathers of three consecutive batches in flight at
+ * once (software pipeline); heavy rows follow over the same grid. Needs
+ * GRG_HIP_FLAG_MERGED or a single segment; unroll 4 or 8; rows_per_group is 1. */
+#define GRG_HIP_FLAG_STREAM 32
 /* Bytes of one staged row (kl * cpl columns) in a tiled launch, at most. */
 #define GRG_HIP_TILE_CHUNK_BYTES 128
 /* Staged rows of one tile, at most (uint16 LDS indices). */
@@ -157,6 +163,9 @@ typedef struct GrgHipGatherArgs {
    * entries over all segments, in segment order, are ekey[eptr[r] ..
    * eptr[r + 1]), each (segment << 30) | source row. */
   const uint32_t* ekey;
+  /* Workgroups of a streamed launch (GRG_HIP_FLAG_STREAM) along the rows, at
+   * most; 0 means as many as are resident at once. */
+  int64_t stream_blocks;
 } GrgHipGatherArgs;
 
 /* One row-gather launch (see above). Replaces one level of the reference's

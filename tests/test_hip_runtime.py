@@ -114,6 +114,43 @@ def test_untuned_launch_shapes(art_a, oracle_a, k):
             _check(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("unroll", [4, 8])
+@pytest.mark.parametrize("es", [1, 4, 16])
+def test_streamed_light_path(art_a, oracle_a, dtype, unroll, es):
+    """The software-pipelined light path (waves looping over row batches, small and
+    capped grids) on every level and write-out launch it applies to, with merged
+    entry keys for the multi-segment down levels and heavy / split rows beside it:
+    exact for +-1 inputs, FP64 within 1e-12, FP32c within 1e-5."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+    from pygrgl_spmv.backends.hip import ffi
+
+    layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
+    shape = {"unroll": unroll, "rows_per_group": 1, "es": es, "flags": ffi.FLAG_STREAM}
+    tol = FP64_TOL if dtype == np.float64 else 1e-5
+    with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16) as rt:
+        (grg,) = rt.grgs
+        rng = np.random.default_rng(3377 + unroll + es)
+        for k in (2, 16):
+            for direction in ("up", "down"):
+                x = rng.choice(np.array([-1.0, 1.0]), size=(k, _cols(oracle_a, direction))).astype(dtype)
+                np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x.astype(np.float64), direction))
+        y = rng.standard_normal((5, oracle_a.num_mutations))
+        miss = rng.standard_normal((5, oracle_a.num_mutations))
+        got = grg.matmul(y.astype(dtype), "down", miss=miss.astype(dtype))
+        assert O.normwise_rel_err(got, O.matmul(oracle_a, y, "down", miss=miss)) <= tol
+        x = rng.standard_normal((8, oracle_a.num_individuals))
+        got = grg.matmul(x.astype(dtype), "up", by_individual=True, init="xtx")
+        assert O.normwise_rel_err(got, O.matmul(oracle_a, x, "up", by_individual=True, init="xtx")) <= tol
+    # a one-workgroup grid: every wave loops over many batches
+    shape["stream_blocks"] = 1
+    with HipRuntime(layout, fixed_shape=shape) as rt:
+        (grg,) = rt.grgs
+        for direction in ("up", "down"):
+            x = rng.standard_normal((3, _cols(oracle_a, direction))).astype(dtype)
+            assert O.normwise_rel_err(grg.matmul(x, direction), O.matmul(oracle_a, x.astype(np.float64), direction)) <= tol
+
+
 @pytest.mark.parametrize("unroll,rows", [(4, 1), (8, 1), (16, 1), (4, 2), (8, 2), (4, 4)])
 @pytest.mark.parametrize("es", [1, 4])
 @pytest.mark.parametrize("merged", [False, True])
