@@ -67,7 +67,8 @@ def cpu_threads() -> int:
     return max(1, n)
 
 
-def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss: bool, calibrate_mib: int = 0):
+def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss: bool, calibrate_mib: int = 0,
+            runtime_kwargs: dict | None = None):
     import torch
 
     from pygrgl_spmv.backends.hip import HipPlanPair, HipRuntime, plan_hip_layout
@@ -78,7 +79,7 @@ def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss:
     layout = plan_hip_layout(artifacts=[artifact], pair=HipPlanPair.default(), dtype=dtype, requirements=req,
                              vram_budget_bytes=0, ring_buffer_size=0, device=0, stream=0)
     t0 = time.perf_counter()
-    with HipRuntime(layout) as rt:
+    with HipRuntime(layout, **(runtime_kwargs or {})) as rt:
         (grg,) = rt.grgs
         up = grg.prepare_matmul_cuda(direction="up", k=k)
         down = grg.prepare_matmul_cuda(direction="down", k=k, use_miss=use_miss)
@@ -133,7 +134,10 @@ def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss:
             torch.cuda.synchronize()
             log(f"calibration copy: {calibrate_mib} MiB read + {calibrate_mib} MiB written")
             del a, b
-    return {"t_up": t_up, "t_down": t_dn, "wall_per_step": wall / steps, "up_out": up_out, "down_out": down_out}
+        tune = [{"config": key[1], "level": key[2], "k": key[3], **{n: (list(v) if isinstance(v, tuple) else v) for n, v in rec.items()}}
+                for key, rec in rt.tune_log.items()]
+    return {"t_up": t_up, "t_down": t_dn, "wall_per_step": wall / steps, "up_out": up_out, "down_out": down_out,
+            "tune": tune}
 
 
 def cpu_baseline(artifact: Path, stats: dict, *, k: int, use_miss: bool, budget_s: float = 20.0):
@@ -212,6 +216,8 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true", help="FP32c only (profiling runs)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after the timed region, run one device copy of this many MiB (for PMC scaling)")
+    ap.add_argument("--tune-log", default="",
+                    help="write the per-level tuner choices and launch times of each form to this JSON file")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
@@ -278,6 +284,10 @@ def main() -> int:
             log(f"cpu baseline: {cpu['value'] / 1e9:.3f} G edge-updates/s on {cpu['cores']} threads")
         except Exception as exc:  # reported, not fatal
             cpu = {"value": None, "unit": "edge-updates/s", "cores": cpu_threads(), "kind": "port", "sample": f"unavailable: {exc}"}
+    if args.tune_log:
+        with open(args.tune_log, "w") as fh:
+            json.dump({"fp64": f64["tune"] if f64 is not None else None,
+                       "fp32c": f32["tune"] if not args.no_fp32c else None}, fh, indent=1)
     head = variants.get("fp64") or variants["fp32c"]
     traffic = load_traffic(args.workload, k)
     out = {

@@ -17,8 +17,9 @@ namespace {
 thread_local std::string g_last_error;
 
 void locality_order(int64_t N, int64_t S, int32_t L, const int64_t* off, const GrgHostBlock* blocks, int32_t nb,
-                    int64_t* new_pos) {
+                    int64_t degree_window, double degree_min_mean, int64_t* new_pos) {
   if (N < 0 || S < 0 || S > N || L < 0) throw std::invalid_argument("bad sizes");
+  if (degree_window < 0) throw std::invalid_argument("degree_window < 0");
   if (off[0] != 0 || off[L] != N) throw std::invalid_argument("level offsets do not cover the nodes");
   // Global children CSR (internal ids), children of a node ordered by source level then column.
   std::vector<int64_t> deg(N + 1, 0);
@@ -93,6 +94,18 @@ void locality_order(int64_t N, int64_t S, int32_t L, const int64_t* off, const G
       std::stable_sort(ids.begin() + (S - lo), ids.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
     } else {
       std::stable_sort(ids.begin(), ids.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+      // Degree windows: inside each run of degree_window consecutive nodes, order by
+      // child count (ties keep key order). A wave sums several consecutive rows in
+      // lockstep rounds until its longest row is done, so rows of similar length
+      // side by side waste fewer rounds; the window keeps the locality of the keys.
+      const int64_t n = hi - lo;
+      if (degree_window > 1 && n > 0 && (double)(deg[hi] - deg[lo]) >= degree_min_mean * (double)n) {
+        for (int64_t w = 0; w < n; w += degree_window) {
+          const int64_t we = std::min(n, w + degree_window);
+          std::stable_sort(ids.begin() + w, ids.begin() + we,
+                           [&](int64_t a, int64_t b) { return deg[a + 1] - deg[a] < deg[b + 1] - deg[b]; });
+        }
+      }
     }
     for (int64_t i = 0; i < hi - lo; ++i) new_pos[ids[i]] = lo + i;
   }
@@ -195,13 +208,15 @@ void build_tiles(int64_t R, int32_t L, const int64_t* off, int32_t nseg, const i
 extern "C" {
 
 int grg_host_locality_order(int64_t num_nodes, int64_t num_samples, int32_t num_levels, const int64_t* level_offsets,
-                            const GrgHostBlock* blocks, int32_t num_blocks, int64_t* new_pos) {
+                            const GrgHostBlock* blocks, int32_t num_blocks, int64_t degree_window, double degree_min_mean,
+                            int64_t* new_pos) {
   if (!level_offsets || !new_pos || (num_blocks > 0 && !blocks)) {
     g_last_error = "null argument";
     return 1;
   }
   try {
-    locality_order(num_nodes, num_samples, num_levels, level_offsets, blocks, num_blocks, new_pos);
+    locality_order(num_nodes, num_samples, num_levels, level_offsets, blocks, num_blocks, degree_window, degree_min_mean,
+                   new_pos);
     return 0;
   } catch (const std::exception& e) {
     g_last_error = e.what();

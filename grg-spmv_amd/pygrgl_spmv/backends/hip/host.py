@@ -25,7 +25,8 @@ def _lib():
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
         lib = ctypes.CDLL(str(LIB_PATH))
-        lib.grg_host_locality_order.argtypes = [c_int64, c_int64, c_int32, c_void_p, ctypes.POINTER(GrgHostBlock), c_int32, c_void_p]
+        lib.grg_host_locality_order.argtypes = [c_int64, c_int64, c_int32, c_void_p, ctypes.POINTER(GrgHostBlock), c_int32,
+                                                c_int64, ctypes.c_double, c_void_p]
         lib.grg_host_locality_order.restype = ctypes.c_int
         lib.grg_host_build_tiles.argtypes = [c_int64, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                              c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -39,8 +40,12 @@ def _lib():
     return _LIB
 
 
-def locality_order(num_nodes: int, num_samples: int, level_offsets: np.ndarray, blocks) -> np.ndarray:
-    """``new_pos[old]`` for blocks given as (dst, src, indptr int64, indices int32) tuples."""
+def locality_order(num_nodes: int, num_samples: int, level_offsets: np.ndarray, blocks, *, degree_window: int = 0,
+                   degree_min_mean: float = 0.0) -> np.ndarray:
+    """``new_pos[old]`` for blocks given as (dst, src, indptr int64, indices int32) tuples.
+
+    ``degree_window > 1`` orders each run of that many consecutive nodes of a level
+    (levels above 0 with mean child count >= ``degree_min_mean``) by child count."""
     lib = _lib()
     off = np.ascontiguousarray(level_offsets, dtype=np.int64)
     keep = []
@@ -51,7 +56,8 @@ def locality_order(num_nodes: int, num_samples: int, level_offsets: np.ndarray, 
         keep.append((ptr, idx))
         arr[i] = GrgHostBlock(int(d), int(s), int(idx.size), ptr.ctypes.data, idx.ctypes.data)
     out = np.empty(int(num_nodes), dtype=np.int64)
-    rc = lib.grg_host_locality_order(int(num_nodes), int(num_samples), int(off.size - 1), off.ctypes.data, arr, len(blocks), out.ctypes.data)
+    rc = lib.grg_host_locality_order(int(num_nodes), int(num_samples), int(off.size - 1), off.ctypes.data, arr, len(blocks),
+                                     int(degree_window), float(degree_min_mean), out.ctypes.data)
     if rc != 0:
         raise RuntimeError(f"grg_host_locality_order failed: status {rc}: {lib.grg_host_last_error().decode()}")
     return out

@@ -71,6 +71,8 @@ KERNEL_UNROLL = 8  # gathers in flight per lane and round in the kernel
 TILE_MAX_STAGED = 128   # staged source rows of one tile (LDS rows)
 TILE_MAX_ROWS = 256     # rows of one tile
 TILE_MAX_ENTRIES = 2048  # gather entries of one tile
+DEGREE_WINDOW = 4096    # nodes per degree-sorted run of the device node order
+DEGREE_MIN_MEAN = 6.0   # levels with a lower mean up-sweep degree keep the plain locality order
 
 
 def column_shapes(dtype_code: int, k: int, lds) -> list[tuple[int, int]]:
@@ -444,7 +446,8 @@ class HipRuntime:
     def __init__(self, layout: HipLayout, *, autotune: bool = True, split_min: int = SPLIT_MIN,
                  split_chunk: int = SPLIT_CHUNK, fixed_shape: dict | None = None, tiles: bool = True,
                  tile_max_staged: int = TILE_MAX_STAGED, tile_max_rows: int = TILE_MAX_ROWS,
-                 tile_max_entries: int = TILE_MAX_ENTRIES) -> None:
+                 tile_max_entries: int = TILE_MAX_ENTRIES, degree_window: int = DEGREE_WINDOW,
+                 degree_min_mean: float = DEGREE_MIN_MEAN) -> None:
         """``autotune``: time the admissible launch shapes of every level launch when an
         op is prepared (otherwise a fixed default shape). ``split_min`` /
         ``split_chunk``: rows of gather degree above ``split_min`` are summed in
@@ -456,7 +459,13 @@ class HipRuntime:
         build the tile tables of the LDS-staged level launches (when the layout
         reserved them) and let the tuner choose them per level; ``tile_max_staged``
         / ``tile_max_rows`` / ``tile_max_entries`` bound the staged sources, rows and
-        gather entries of one tile."""
+        gather entries of one tile. ``degree_window`` / ``degree_min_mean``: the
+        device node order sorts each run of ``degree_window`` consecutive nodes of a
+        level (levels whose mean gather degree in the up sweep is at least
+        ``degree_min_mean``) by degree, so the rows a wave sums in lockstep have
+        similar lengths (0 keeps the plain locality order)."""
+        if int(degree_window) < 0:
+            raise ValueError("degree_window must be >= 0")
         if int(split_min) < 1 or int(split_chunk) < 1:
             raise ValueError("split_min and split_chunk must be >= 1")
         torch = _torch()
@@ -479,6 +488,8 @@ class HipRuntime:
         self.tile_max_staged = int(tile_max_staged)
         self.tile_max_rows = int(tile_max_rows)
         self.tile_max_entries = int(tile_max_entries)
+        self.degree_window = int(degree_window)
+        self.degree_min_mean = float(degree_min_mean)
         self._dtype_code = ffi.F64 if np.dtype(layout.dtype) == np.float64 else ffi.F32C
         self._reset_owned_state()
 
@@ -695,7 +706,8 @@ class HipRuntime:
             host_blocks.append((blk.dst_level, blk.src_level, np.asarray(blk.indptr, dtype=np.int64), idx))
         # Device-private node order: a permutation inside every level that keeps nodes
         # sharing descendants adjacent (samples stay the prefix of level 0).
-        new_pos = locality_order(N, S, off, host_blocks)
+        new_pos = locality_order(N, S, off, host_blocks, degree_window=self.degree_window,
+                                 degree_min_mean=self.degree_min_mean)
         new_pos_t = self._to_dev(new_pos)
         up_parts, dn_parts = [], []
         up_stream = [[] for _ in range(L)]
@@ -1021,7 +1033,7 @@ class HipRuntime:
             untiled = [t for t in timings if not t[6] & ffi.FLAG_TILED]
             self.tune_log[key] = {"rows": int(a.row_end - a.row_begin), "best": min(timings),
                                   "untiled_ms": min(untiled)[0] if untiled else None,
-                                  "tiled_ms": min(tiled)[0] if tiled else None}
+                                  "tiled_ms": min(tiled)[0] if tiled else None, "timings": sorted(timings)}
         if best is not None:
             a.cpl, a.kl, a.es, a.unroll, a.rows_per_group, a.flags = best
 

@@ -28,10 +28,14 @@ typedef struct GrgHostBlock {
  * level (level offsets are unchanged). Samples 0..num_samples-1 stay in
  * [0, num_samples) ordered by first visit of a depth-first walk from the top
  * levels; every other node is keyed by the mean key of its children (bottom
- * up), childless non-sample nodes go last in their level. Returns 0 on
- * success. */
+ * up), childless non-sample nodes go last in their level. With
+ * degree_window > 1, every level above 0 whose mean child count is at least
+ * degree_min_mean is then cut into runs of degree_window consecutive nodes,
+ * each run ordered by child count (stable), so that the rows a wave sums in
+ * lockstep have similar lengths. Returns 0 on success. */
 int grg_host_locality_order(int64_t num_nodes, int64_t num_samples, int32_t num_levels, const int64_t* level_offsets,
-                            const GrgHostBlock* blocks, int32_t num_blocks, int64_t* new_pos);
+                            const GrgHostBlock* blocks, int32_t num_blocks, int64_t degree_window, double degree_min_mean,
+                            int64_t* new_pos);
 
 /* Tiles of consecutive rows for the tiled launch of libgrg_hip.so (see the
  * tile fields of GrgHipGatherArgs in grg_hip.h), for every level of one sweep
