@@ -835,6 +835,242 @@ __global__ __launch_bounds__(kBlock) void tile_gather_kernel(const GrgHipGatherA
   }
 }
 
+// ---- panel launch -----------------------------------------------------------
+
+// LDS of a panel workgroup, in bytes: the panel's pass entry offsets, one
+// pass's staged rows, its row offsets (uint16) and its entry positions (uint8,
+// plus 32 bytes of slack for the 16-byte word alignment of the copy).
+struct PanelLds {
+  int64_t meta, staged, rowoff, entries;
+  __host__ __device__ int64_t total() const { return meta + staged + rowoff + entries; }
+};
+
+__host__ __device__ inline PanelLds panel_lds_layout(int64_t max_passes, int64_t staged_rows, int64_t stride_bytes,
+                                                     int64_t panel_rows, int64_t max_entries) {
+  PanelLds t;
+  t.meta = round16((max_passes + 1) * 4);
+  t.staged = round16(staged_rows * stride_bytes);
+  t.rowoff = round16((panel_rows + 1) * 2);
+  t.entries = round16(max_entries + 32);
+  return t;
+}
+
+constexpr int kPanelWords = 4;  // 16-byte words of entry positions per thread and pass, at most
+
+// Staged pieces (kl-lane vectors) of pass q of a panel whose sources are
+// pan_src[sbase .. send): every pass but the last stages panel_staged rows.
+__device__ __forceinline__ int panel_pieces(const GrgHipGatherArgs& a, int32_t sbase, int32_t send, int q, int kl_shift) {
+  const int32_t s0 = sbase + q * a.panel_staged;
+  const int32_t n = send - s0;
+  return (n < a.panel_staged ? n : a.panel_staged) << kl_shift;
+}
+
+template <int PPT>
+__device__ __forceinline__ void panel_keys(const GrgHipGatherArgs& a, int32_t sbase, int32_t send, int q, int kl_shift,
+                                           uint32_t (&key)[PPT]) {
+  const int32_t s0 = sbase + q * a.panel_staged;
+  const int n_pieces = panel_pieces(a, sbase, send, q, kl_shift);
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int i = threadIdx.x + u * kBlock;
+    key[u] = i < n_pieces ? a.pan_src[s0 + (i >> kl_shift)] : 0xffffffffu;
+  }
+}
+
+// Source row pieces of one pass, by key, into registers.
+template <typename T, int CPL, int PPT>
+__device__ __forceinline__ void panel_rows_load(const GrgHipGatherArgs& a, const uint32_t (&key)[PPT], int kl_shift, int col0,
+                                                T (&v)[PPT][CPL]) {
+  const int KL = 1 << kl_shift;
+  const T* src0 = static_cast<const T*>(a.seg[0].src);
+  const T* src1 = a.nseg > 1 ? static_cast<const T*>(a.seg[1].src) : src0;
+  const T* src2 = a.nseg > 2 ? static_cast<const T*>(a.seg[2].src) : src0;
+  const int64_t ld0 = a.seg[0].ld;
+  const int64_t ld1 = a.nseg > 1 ? a.seg[1].ld : ld0;
+  const int64_t ld2 = a.nseg > 2 ? a.seg[2].ld : ld0;
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int i = threadIdx.x + u * kBlock;
+    const int col = col0 + (i & (KL - 1)) * CPL;
+    if (key[u] != 0xffffffffu && col < a.k) {
+      const uint32_t sg = key[u] >> 30;
+      const int64_t srow = (int64_t)(key[u] & 0x3fffffffu);
+      const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
+      const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
+      load_raw<T, CPL>(base + srow * ld + col, v[u]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) v[u][c] = T(0);
+    }
+  }
+}
+
+typedef uint32_t PanelWord __attribute__((ext_vector_type(4)));
+
+// Pass g's row offsets and its entry positions [e0, e1), as 16-byte words, into registers.
+__device__ __forceinline__ void panel_record_load(const GrgHipGatherArgs& a, int64_t g, int32_t e0, int32_t e1, int ent_words,
+                                                  PanelWord (&w)[kPanelWords], uint16_t& roff) {
+  const int R = a.panel_rows;
+  const int32_t w0 = e0 >> 4;
+  int32_t nw = ((e1 + 15) >> 4) - w0;
+  if (nw > ent_words) nw = ent_words;  // never more than the LDS holds
+  const PanelWord* src = reinterpret_cast<const PanelWord*>(a.pan_lidx) + w0;
+#pragma unroll
+  for (int u = 0; u < kPanelWords; ++u) {
+    const int i = threadIdx.x + u * kBlock;
+    w[u] = i < nw ? src[i] : PanelWord{0u, 0u, 0u, 0u};
+  }
+  roff = threadIdx.x <= R ? a.pass_rowoff[g * (R + 1) + threadIdx.x] : (uint16_t)0;
+}
+
+template <typename T, int CPL, int PPT>
+__device__ __forceinline__ void panel_store_pass(const GrgHipGatherArgs& a, int n_pieces, int kl_shift, int stride,
+                                                 const T (&v)[PPT][CPL], const PanelWord (&w)[kPanelWords], uint16_t roff,
+                                                 int ent_words, T* buf, uint16_t* rowoff, PanelWord* ent) {
+  const int KL = 1 << kl_shift;
+#pragma unroll
+  for (int u = 0; u < PPT; ++u) {
+    const int i = threadIdx.x + u * kBlock;
+    if (i < n_pieces) store_raw<T, CPL>(buf + (i >> kl_shift) * stride + (i & (KL - 1)) * CPL, v[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < kPanelWords; ++u) {
+    const int i = threadIdx.x + u * kBlock;
+    if (i < ent_words) ent[i] = w[u];  // the entry region holds ent_words words
+  }
+  if (threadIdx.x <= a.panel_rows) rowoff[threadIdx.x] = roff;
+}
+
+// Panel launch: workgroup t owns panel t (panel_rows consecutive rows, one per
+// lane group). The panel's pass entry offsets are read into LDS once; each
+// pass then stages panel_staged of the panel's distinct source rows in LDS
+// with the pass's entry lists, while the next pass's source rows, entry lists
+// and the following pass's keys are in flight. Each lane group keeps its row's
+// sums in registers over all passes and writes the row once. Heavy rows follow
+// as ordinary workgroup rows.
+template <typename T, int CPL, int PPT>
+__global__ __launch_bounds__(kBlock) void panel_gather_kernel(const GrgHipGatherArgs a, int g_shift, int kl_shift) {
+  using Acc = typename SumOf<T>::type;
+  extern __shared__ __align__(16) unsigned char panel_lds[];
+  __shared__ Acc partial[kWavesPerBlock][64];
+  const int lane = threadIdx.x & 63;
+  const int KL = 1 << kl_shift;
+  const int64_t np = a.num_panels;
+  int64_t t = blockIdx.x;
+  if (t >= np) {
+    workgroup_row<T, CPL, 8>(a, a.heavy_rows[t - np], t - np, (int)blockIdx.y, kl_shift, partial);
+    return;
+  }
+  if (a.flags & GRG_HIP_FLAG_XCD_REMAP) {
+    const int64_t q = np >> 3, r = np & 7, x = t & 7;
+    t = x * q + (x < r ? x : r) + (t >> 3);
+  }
+  const int R = a.panel_rows;
+  const int chunk = KL * CPL;
+  const int stride = chunk + tile_pad<T, CPL>();
+  const int col0 = (int)blockIdx.y * chunk;
+  const PanelLds lay = panel_lds_layout(a.panel_max_passes, a.panel_staged, (int64_t)stride * (int64_t)sizeof(T), R,
+                                        a.panel_max_entries);
+  int32_t* meta = reinterpret_cast<int32_t*>(panel_lds);
+  T* buf = reinterpret_cast<T*>(panel_lds + lay.meta);
+  uint16_t* rowoff = reinterpret_cast<uint16_t*>(panel_lds + lay.meta + lay.staged);
+  PanelWord* ent_w = reinterpret_cast<PanelWord*>(panel_lds + lay.meta + lay.staged + lay.rowoff);
+  const uint8_t* ent = reinterpret_cast<const uint8_t*>(ent_w);
+  const int ent_words = (int)(lay.entries >> 4);
+
+  const int G = 1 << g_shift;
+  const int lg = lane & (G - 1);
+  const int cl = lg & (KL - 1);
+  const int e = lg >> kl_shift;
+  const int ES = G >> kl_shift;
+  const int gid = (int)threadIdx.x >> g_shift;
+  const int col = col0 + cl * CPL;
+  const int64_t row = a.row_begin + t * R + gid;
+  bool live = row < a.row_end && col < a.k;
+  if (live && a.heavy_threshold >= 0) {
+    int64_t deg = 0;
+    for (int s = 0; s < a.nseg; ++s) {
+      const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
+      deg += ptr[1] - ptr[0];
+    }
+    live = deg <= a.heavy_threshold;  // a heavy row is produced by its heavy workgroup
+  }
+  const T* lds_col = buf + cl * CPL;
+  Acc acc[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) acc[c].zero();
+
+  // the panel's passes, their sources pan_src[sbase .. send) and entry offsets (in LDS)
+  const int32_t g0 = a.pan_pass[t];
+  int npass = a.pan_pass[t + 1] - g0;
+  if (npass > a.panel_max_passes) npass = a.panel_max_passes;  // never more than the LDS holds
+  const int32_t sbase = a.pass_src[g0], send = a.pass_src[g0 + npass];
+  for (int i = threadIdx.x; i <= npass; i += kBlock) meta[i] = a.pass_ent[g0 + i];
+  __syncthreads();
+
+  uint32_t key[PPT];
+  T v[PPT][CPL];
+  PanelWord w[kPanelWords];
+  uint16_t roff = 0;
+  if (npass > 0) {
+    panel_keys<PPT>(a, sbase, send, 0, kl_shift, key);
+    panel_record_load(a, g0, meta[0], meta[1], ent_words, w, roff);
+    panel_rows_load<T, CPL, PPT>(a, key, kl_shift, col0, v);
+    panel_store_pass<T, CPL, PPT>(a, panel_pieces(a, sbase, send, 0, kl_shift), kl_shift, stride, v, w, roff, ent_words, buf,
+                                  rowoff, ent_w);
+    if (npass > 1) panel_keys<PPT>(a, sbase, send, 1, kl_shift, key);
+  }
+  __syncthreads();
+  for (int q = 0; q < npass; ++q) {
+    // loads of pass q + 1 (and the keys of pass q + 2) fly while pass q is summed
+    if (q + 1 < npass) {
+      panel_rows_load<T, CPL, PPT>(a, key, kl_shift, col0, v);
+      panel_record_load(a, g0 + q + 1, meta[q + 1], meta[q + 2], ent_words, w, roff);
+      if (q + 2 < npass) panel_keys<PPT>(a, sbase, send, q + 2, kl_shift, key);
+    }
+    if (live) {
+      const int p0 = rowoff[gid], p1 = rowoff[gid + 1];
+      const uint8_t* ent_q = ent + (meta[q] & 15);
+      for (int j = p0 + e; j < p1; j += 4 * ES) {
+        int li[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int jj = j + u * ES;
+          li[u] = jj < p1 ? (int)ent_q[jj] : -1;
+        }
+        T x[4][CPL];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (li[u] >= 0) {
+            load_raw<T, CPL>(lds_col + li[u] * stride, x[u]);
+          } else {
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) x[u][c] = T(0);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) acc[c].add(x[u][c]);
+      }
+    }
+    __syncthreads();
+    if (q + 1 < npass)
+      panel_store_pass<T, CPL, PPT>(a, panel_pieces(a, sbase, send, q + 1, kl_shift), kl_shift, stride, v, w, roff, ent_words,
+                                    buf, rowoff, ent_w);
+    __syncthreads();
+  }
+  for (int off = KL; off < G; off <<= 1) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) acc[c].merge_xor(off);
+  }
+  if (live && e == 0) {
+    add_row_terms<T, CPL>(a, row, col, acc);
+    store_row<T, CPL>(a, row, col, acc);
+  }
+}
+
+
 // ---- job tables -------------------------------------------------------------
 
 constexpr uint32_t kSpinLimit = 1u << 24;  // barrier polls before a wait is declared failed
@@ -942,6 +1178,16 @@ int64_t tile_lds_bytes(const GrgHipGatherArgs& a) {
   return tile_lds_layout(a.tile_lds_rows, stride * elem, a.tile_max_rows, a.tile_max_entries).total() + stat;
 }
 
+// Dynamic LDS of a panel launch plus the kernel's static LDS.
+int64_t panel_lds_bytes(const GrgHipGatherArgs& a) {
+  const bool f64 = a.dtype == GRG_HIP_F64;
+  const int64_t elem = f64 ? 8 : 4;
+  const int64_t pad = a.cpl * elem >= 16 ? a.cpl : 16 / elem;
+  const int64_t stride = (int64_t)a.kl * a.cpl + pad;
+  const int64_t stat = f64 ? tile_static_lds<double>() : tile_static_lds<float>();
+  return panel_lds_layout(a.panel_max_passes, a.panel_staged, stride * elem, a.panel_rows, a.panel_max_entries).total() + stat;
+}
+
 int validate(const GrgHipGatherArgs& a) {
   if (a.dtype != GRG_HIP_F64 && a.dtype != GRG_HIP_F32C) return fail("unknown dtype code " + std::to_string(a.dtype));
   if (a.row_end < a.row_begin) return fail("row_end < row_begin");
@@ -957,8 +1203,26 @@ int validate(const GrgHipGatherArgs& a) {
     if (!ok) return fail("(unroll, rows_per_group) must be one of (4|8|16, 1), (4|8, 2), (4, 4)");
   }
   if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_MERGED |
-                  GRG_HIP_FLAG_STREAM))
+                  GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_PANEL))
     return fail("unknown flags");
+  if (a.flags & GRG_HIP_FLAG_PANEL) {
+    if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_MERGED))
+      return fail("a panel launch is not tiled, merged, streamed or a partials launch");
+    if (!a.pan_pass || !a.pass_src || !a.pass_ent || !a.pass_rowoff || !a.pan_src || !a.pan_lidx)
+      return fail("panel launch needs pan_pass, pass_src, pass_ent, pass_rowoff, pan_src and pan_lidx");
+    if (a.nseg < 1) return fail("panel launch needs at least one segment");
+    if (a.num_panels < 0) return fail("num_panels < 0");
+    if ((int64_t)a.panel_rows * a.kl * a.es != kBlock || a.panel_rows > kBlock / 2)
+      return fail("panel launch: panel_rows * kl * es must be 256, with kl * es >= 2");
+    if (a.num_panels * (int64_t)a.panel_rows < a.row_end - a.row_begin) return fail("panel launch: the panels do not cover the rows");
+    if (a.panel_staged < 1 || a.panel_staged > 256 || (int64_t)a.panel_staged * a.kl > 4 * (int64_t)kBlock)
+      return fail("panel launch: panel_staged must be in [1, 256] with panel_staged * kl <= 1024");
+    if (a.panel_max_passes < 0) return fail("panel launch: panel_max_passes < 0");
+    if (a.panel_max_entries < 0 || (int64_t)a.panel_max_entries + 32 > (int64_t)kPanelWords * kBlock * 16)
+      return fail("panel launch: panel_max_entries out of range");
+    if (reinterpret_cast<uintptr_t>(a.pan_lidx) % 16) return fail("pan_lidx must be 16-byte aligned");
+    if (panel_lds_bytes(a) > kTileLdsLimit) return fail("panel launch: one pass exceeds the LDS");
+  }
   if (a.flags & GRG_HIP_FLAG_STREAM) {
     if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PARTIALS)) return fail("a streamed launch is neither tiled nor a partials launch");
     if (!(a.flags & GRG_HIP_FLAG_MERGED) && a.nseg != 1) return fail("a streamed launch needs merged entries or one segment");
@@ -1037,6 +1301,27 @@ int launch_tiled_typed(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl
   return 0;
 }
 
+template <typename T, int CPL, int PPT>
+int launch_panel_pieces(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl_shift, hipStream_t s) {
+  const size_t dyn = (size_t)(panel_lds_bytes(a) - tile_static_lds<T>());
+  if (dyn > 64 * 1024) {
+    hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_gather_kernel<T, CPL, PPT>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (err != hipSuccess) return fail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(err), 2);
+  }
+  hipLaunchKernelGGL((panel_gather_kernel<T, CPL, PPT>), grid, dim3(kBlock), dyn, s, a, g_shift, kl_shift);
+  return 0;
+}
+
+// pieces staged per thread and pass: panel_staged * kl / 256, rounded up to 1, 2 or 4
+template <typename T, int CPL>
+int launch_panel_typed(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl_shift, hipStream_t s) {
+  const int64_t pieces = ((int64_t)a.panel_staged * a.kl + kBlock - 1) / kBlock;
+  if (pieces <= 1) return launch_panel_pieces<T, CPL, 1>(a, grid, g_shift, kl_shift, s);
+  if (pieces <= 2) return launch_panel_pieces<T, CPL, 2>(a, grid, g_shift, kl_shift, s);
+  return launch_panel_pieces<T, CPL, 4>(a, grid, g_shift, kl_shift, s);
+}
+
 template <typename T, int CPL>
 int launch_stream_typed(const GrgHipGatherArgs& a, int64_t want_blocks, int gy, int g_shift, int kl_shift, hipStream_t s) {
   int dev = 0, cus = 0, per_cu = 0;
@@ -1082,6 +1367,32 @@ int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
         case 2: rc = launch_stream_typed<float, 2>(a, want, gy, g_shift, kl_shift, stream); break;
         case 8: rc = launch_stream_typed<float, 8>(a, want, gy, g_shift, kl_shift, stream); break;
         default: rc = launch_stream_typed<float, 4>(a, want, gy, g_shift, kl_shift, stream); break;
+      }
+    }
+    if (rc) return rc;
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return fail(std::string("kernel launch failed: ") + hipGetErrorString(err), 2);
+    return 0;
+  }
+  if (a.flags & GRG_HIP_FLAG_PANEL) {
+    gx = a.num_panels + a.num_heavy;
+    if (gx == 0) return 0;
+    if (gx > 0x7fffffffLL) return fail("grid too large");
+    const int chunk = a.kl * a.cpl;
+    dim3 grid((unsigned)gx, (unsigned)((a.k + chunk - 1) / chunk));
+    int rc = 0;
+    if (a.dtype == GRG_HIP_F64) {
+      switch (a.cpl) {
+        case 1: rc = launch_panel_typed<double, 1>(a, grid, g_shift, kl_shift, stream); break;
+        case 2: rc = launch_panel_typed<double, 2>(a, grid, g_shift, kl_shift, stream); break;
+        default: rc = launch_panel_typed<double, 4>(a, grid, g_shift, kl_shift, stream); break;
+      }
+    } else {
+      switch (a.cpl) {
+        case 1: rc = launch_panel_typed<float, 1>(a, grid, g_shift, kl_shift, stream); break;
+        case 2: rc = launch_panel_typed<float, 2>(a, grid, g_shift, kl_shift, stream); break;
+        case 8: rc = launch_panel_typed<float, 8>(a, grid, g_shift, kl_shift, stream); break;
+        default: rc = launch_panel_typed<float, 4>(a, grid, g_shift, kl_shift, stream); break;
       }
     }
     if (rc) return rc;
@@ -1181,7 +1492,8 @@ int grg_hip_jobs_build(const GrgHipGatherArgs* args, const int32_t* phase_of, in
       g_last_error = "job " + std::to_string(i) + ": " + g_last_error;
       return rc;
     }
-    if (a.flags & GRG_HIP_FLAG_TILED) return fail("job " + std::to_string(i) + ": tiled launches are not run from job tables");
+    if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM))
+      return fail("job " + std::to_string(i) + ": tiled, panel and streamed launches are not run from job tables");
     if (a.dtype != args[0].dtype) return fail("jobs of one table share one value type");
     const int32_t p = phase_of[i];
     if (p < 0 || p >= nphase || p < cur) return fail("phase_of must be non-decreasing and in [0, nphase)");

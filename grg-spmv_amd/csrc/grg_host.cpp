@@ -8,6 +8,8 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/grg_host.h"
@@ -203,6 +205,135 @@ void build_tiles(int64_t R, int32_t L, const int64_t* off, int32_t nseg, const i
   if (ns > INT32_MAX) throw std::invalid_argument("more than 2^31 - 1 staged sources");
 }
 
+
+// Panels of the panel launch: runs of panel_rows consecutive rows whose
+// distinct sources (the union of their entries' keys, ascending) are staged
+// max_staged at a time ("passes"). Per pass, every row's entries that fall in
+// the staged window are listed as 8-bit window positions, ascending.
+struct PanelSet {
+  int32_t panel_rows = 0;
+  int32_t max_staged = 0;
+  int32_t max_pass_entries = 0;
+  int32_t max_passes = 0;
+  std::vector<int32_t> pan_pass{0};
+  std::vector<int32_t> pass_src{0};
+  std::vector<int32_t> pass_ent{0};
+  std::vector<uint16_t> pass_rowoff;
+  std::vector<uint32_t> src_list;
+  std::vector<uint8_t> lidx;
+};
+
+struct PanelPart {  // panels [p0, p1) built by one thread, offsets local to the part
+  std::vector<int32_t> n_pass;  // passes per panel
+  std::vector<int32_t> pass_src, pass_ent;  // per pass: sources, entries
+  std::vector<uint16_t> pass_rowoff;
+  std::vector<uint32_t> src_list;
+  std::vector<uint8_t> lidx;
+  int32_t max_pass_entries = 0;
+};
+
+void build_panel_part(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys, int32_t R, int32_t P,
+                      int64_t heavy, int64_t p0, int64_t p1, PanelPart& out) {
+  std::vector<std::pair<uint32_t, int32_t>> ent;  // (key, row in panel)
+  std::vector<uint32_t> uni;
+  std::vector<int32_t> cnt;
+  for (int64_t p = p0; p < p1; ++p) {
+    const int64_t r0 = row_begin + p * R, r1 = std::min<int64_t>(row_end, r0 + R);
+    ent.clear();
+    for (int64_t r = r0; r < r1; ++r) {
+      const int64_t deg = (int64_t)eptr[r + 1] - eptr[r];
+      if (deg < 0) throw std::invalid_argument("eptr is not non-decreasing");
+      if (heavy >= 0 && deg > heavy) continue;  // produced by its heavy workgroup
+      for (int64_t j = eptr[r]; j < eptr[r + 1]; ++j) ent.emplace_back(keys[j], (int32_t)(r - r0));
+    }
+    std::sort(ent.begin(), ent.end());
+    uni.clear();
+    for (size_t i = 0; i < ent.size(); ++i) {
+      if (i > 0 && ent[i].first == ent[i - 1].first && ent[i].second == ent[i - 1].second)
+        throw std::invalid_argument("a row lists the same source twice");
+      if (uni.empty() || uni.back() != ent[i].first) uni.push_back(ent[i].first);
+    }
+    const int32_t np = (int32_t)((uni.size() + P - 1) / P);
+    out.n_pass.push_back(np);
+    out.src_list.insert(out.src_list.end(), uni.begin(), uni.end());
+    // entries bucketed by (pass, row); within a bucket in ascending position
+    cnt.assign((size_t)np * R, 0);
+    size_t u = 0;
+    std::vector<int32_t> pos(ent.size());
+    for (size_t i = 0; i < ent.size(); ++i) {
+      while (uni[u] != ent[i].first) ++u;
+      pos[i] = (int32_t)u;
+      ++cnt[(size_t)(u / P) * R + ent[i].second];
+    }
+    const size_t base = out.lidx.size();
+    std::vector<int64_t> fill((size_t)np * R);
+    int64_t acc = 0;
+    for (int32_t q = 0; q < np; ++q) {
+      out.pass_src.push_back((int32_t)std::min<int64_t>(P, (int64_t)uni.size() - (int64_t)q * P));
+      const int64_t q0 = acc;
+      for (int32_t r = 0; r < R; ++r) {
+        out.pass_rowoff.push_back((uint16_t)(acc - q0));
+        fill[(size_t)q * R + r] = acc;
+        acc += cnt[(size_t)q * R + r];
+      }
+      out.pass_rowoff.push_back((uint16_t)(acc - q0));
+      if (acc - q0 > 65535) throw std::invalid_argument("a pass holds more than 65535 entries");
+      out.pass_ent.push_back((int32_t)(acc - q0));
+      out.max_pass_entries = std::max<int32_t>(out.max_pass_entries, (int32_t)(acc - q0));
+    }
+    out.lidx.resize(base + (size_t)acc);
+    for (size_t i = 0; i < ent.size(); ++i) {
+      const int32_t q = pos[i] / P;
+      out.lidx[base + (size_t)fill[(size_t)q * R + ent[i].second]++] = (uint8_t)(pos[i] - q * P);
+    }
+  }
+}
+
+PanelSet* build_panels(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys, int32_t R, int32_t P,
+                       int64_t heavy, int32_t threads) {
+  if (row_end < row_begin || row_begin < 0) throw std::invalid_argument("bad row range");
+  if (R < 1 || R > 256 || P < 1 || P > 256) throw std::invalid_argument("panel_rows and max_staged must be in [1, 256]");
+  const int64_t n_panels = (row_end - row_begin + R - 1) / R;
+  const int64_t T = std::max<int64_t>(1, std::min<int64_t>(threads > 0 ? threads : 1, n_panels));
+  std::vector<PanelPart> parts((size_t)T);
+  std::vector<std::string> errs((size_t)T);
+  std::vector<std::thread> pool;
+  for (int64_t t = 0; t < T; ++t) {
+    pool.emplace_back([&, t]() {
+      try {
+        build_panel_part(row_begin, row_end, eptr, keys, R, P, heavy, n_panels * t / T, n_panels * (t + 1) / T, parts[(size_t)t]);
+      } catch (const std::exception& e) {
+        errs[(size_t)t] = e.what();
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  for (const auto& e : errs)
+    if (!e.empty()) throw std::invalid_argument(e);
+  auto* ps = new PanelSet();
+  ps->panel_rows = R;
+  ps->max_staged = P;
+  for (auto& part : parts) {
+    int64_t q = 0;
+    for (int32_t np : part.n_pass) {
+      ps->max_passes = std::max(ps->max_passes, np);
+      ps->pan_pass.push_back(ps->pan_pass.back() + np);
+      for (int32_t i = 0; i < np; ++i, ++q) {
+        const int64_t s = (int64_t)ps->pass_src.back() + part.pass_src[(size_t)q];
+        const int64_t e = (int64_t)ps->pass_ent.back() + part.pass_ent[(size_t)q];
+        if (s > INT32_MAX || e > INT32_MAX) throw std::invalid_argument("more than 2^31 - 1 panel sources or entries");
+        ps->pass_src.push_back((int32_t)s);
+        ps->pass_ent.push_back((int32_t)e);
+      }
+    }
+    ps->pass_rowoff.insert(ps->pass_rowoff.end(), part.pass_rowoff.begin(), part.pass_rowoff.end());
+    ps->src_list.insert(ps->src_list.end(), part.src_list.begin(), part.src_list.end());
+    ps->lidx.insert(ps->lidx.end(), part.lidx.begin(), part.lidx.end());
+    ps->max_pass_entries = std::max(ps->max_pass_entries, part.max_pass_entries);
+  }
+  return ps;
+}
+
 }  // namespace
 
 extern "C" {
@@ -275,5 +406,56 @@ int grg_host_merged_keys(int64_t num_rows, int32_t nseg, const int32_t* const* s
 }
 
 const char* grg_host_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"
+
+extern "C" {
+
+void* grg_host_build_panels(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys, int32_t panel_rows,
+                            int32_t max_staged, int64_t heavy_threshold, int32_t threads) {
+  if (!eptr || (!keys && row_end > row_begin)) {
+    g_last_error = "null argument";
+    return nullptr;
+  }
+  try {
+    return build_panels(row_begin, row_end, eptr, keys, panel_rows, max_staged, heavy_threshold, threads);
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return nullptr;
+  }
+}
+
+int grg_host_panels_sizes(const void* panels, int64_t* sizes) {
+  if (!panels || !sizes) {
+    g_last_error = "null argument";
+    return 1;
+  }
+  const PanelSet* ps = static_cast<const PanelSet*>(panels);
+  sizes[0] = (int64_t)ps->pan_pass.size() - 1;
+  sizes[1] = (int64_t)ps->pass_src.size() - 1;
+  sizes[2] = (int64_t)ps->src_list.size();
+  sizes[3] = (int64_t)ps->lidx.size();
+  sizes[4] = ps->max_pass_entries;
+  sizes[5] = ps->max_passes;
+  return 0;
+}
+
+int grg_host_panels_copy(const void* panels, int32_t* pan_pass, int32_t* pass_src, int32_t* pass_ent, uint16_t* pass_rowoff,
+                         uint32_t* src_list, uint8_t* lidx) {
+  if (!panels || !pan_pass || !pass_src || !pass_ent || !pass_rowoff || !src_list || !lidx) {
+    g_last_error = "null argument";
+    return 1;
+  }
+  const PanelSet* ps = static_cast<const PanelSet*>(panels);
+  std::copy(ps->pan_pass.begin(), ps->pan_pass.end(), pan_pass);
+  std::copy(ps->pass_src.begin(), ps->pass_src.end(), pass_src);
+  std::copy(ps->pass_ent.begin(), ps->pass_ent.end(), pass_ent);
+  std::copy(ps->pass_rowoff.begin(), ps->pass_rowoff.end(), pass_rowoff);
+  std::copy(ps->src_list.begin(), ps->src_list.end(), src_list);
+  std::copy(ps->lidx.begin(), ps->lidx.end(), lidx);
+  return 0;
+}
+
+void grg_host_panels_free(void* panels) { delete static_cast<PanelSet*>(panels); }
 
 }  // extern "C"

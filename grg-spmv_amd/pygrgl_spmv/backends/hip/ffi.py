@@ -13,7 +13,7 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 10
+ABI_VERSION = 11
 F64 = 0
 F32C = 1
 FLAG_XCD_REMAP = 2
@@ -21,6 +21,7 @@ FLAG_PARTIALS = 4
 FLAG_TILED = 8
 FLAG_MERGED = 16
 FLAG_STREAM = 32
+FLAG_PANEL = 64
 TILE_CHUNK_BYTES = 128
 TILE_MAX_STAGED = 1024
 MAX_SEGS = 3
@@ -79,6 +80,17 @@ class GrgHipGatherArgs(ctypes.Structure):
         ("tile_max_entries", c_int64),
         ("ekey", c_void_p),
         ("stream_blocks", c_int64),
+        ("pan_pass", c_void_p),
+        ("pass_src", c_void_p),
+        ("pass_ent", c_void_p),
+        ("pass_rowoff", c_void_p),
+        ("pan_src", c_void_p),
+        ("pan_lidx", c_void_p),
+        ("num_panels", c_int64),
+        ("panel_rows", c_int32),
+        ("panel_staged", c_int32),
+        ("panel_max_entries", c_int32),
+        ("panel_max_passes", c_int32),
     ]
 
 

@@ -34,6 +34,14 @@ def _lib():
         lib.grg_host_build_tiles.restype = ctypes.c_int
         lib.grg_host_merged_keys.argtypes = [c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]
         lib.grg_host_merged_keys.restype = ctypes.c_int
+        lib.grg_host_build_panels.argtypes = [c_int64, c_int64, c_void_p, c_void_p, c_int32, c_int32, c_int64, c_int32]
+        lib.grg_host_build_panels.restype = c_void_p
+        lib.grg_host_panels_sizes.argtypes = [c_void_p, c_void_p]
+        lib.grg_host_panels_sizes.restype = ctypes.c_int
+        lib.grg_host_panels_copy.argtypes = [c_void_p] * 7
+        lib.grg_host_panels_copy.restype = ctypes.c_int
+        lib.grg_host_panels_free.argtypes = [c_void_p]
+        lib.grg_host_panels_free.restype = None
         lib.grg_host_last_error.argtypes = []
         lib.grg_host_last_error.restype = ctypes.c_char_p
         _LIB = lib
@@ -159,3 +167,70 @@ def merged_keys(eptr: np.ndarray, segments) -> np.ndarray:
 
 
 __all__ = ["LIB_PATH", "LIDX_PAD", "Tiles", "build_tiles", "locality_order", "merged_keys"]
+
+
+@dataclass
+class Panels:
+    """Panels of one level launch (``grg_host_build_panels``): int32 ``pan_pass``
+    (panels + 1), ``pass_src`` / ``pass_ent`` (passes + 1), uint16 ``pass_rowoff``
+    (passes x (panel_rows + 1)), uint32 ``src_list`` and uint8 ``lidx`` (each padded
+    with PANEL_PAD zeros so 16-byte reads past the end stay inside)."""
+
+    panel_rows: int
+    max_staged: int
+    max_pass_entries: int
+    max_passes: int
+    pan_pass: np.ndarray
+    pass_src: np.ndarray
+    pass_ent: np.ndarray
+    pass_rowoff: np.ndarray
+    src_list: np.ndarray
+    lidx: np.ndarray
+
+    @property
+    def num_panels(self) -> int:
+        return int(self.pan_pass.size - 1)
+
+    @property
+    def nbytes(self) -> int:
+        return int(sum(a.nbytes for a in (self.pan_pass, self.pass_src, self.pass_ent, self.pass_rowoff, self.src_list,
+                                          self.lidx)))
+
+
+PANEL_PAD = 16
+
+
+def build_panels(eptr: np.ndarray, keys: np.ndarray, row_begin: int, row_end: int, *, panel_rows: int = 64,
+                 max_staged: int = 128, heavy_threshold: int = -1, threads: int = 0) -> Panels:
+    """Panels for rows [row_begin, row_end) of entries ``keys[eptr[r]:eptr[r + 1]]``."""
+    import os
+
+    lib = _lib()
+    eptr = np.ascontiguousarray(eptr, dtype=np.int32)
+    keys = np.ascontiguousarray(keys).view(np.uint32) if np.asarray(keys).dtype.itemsize == 4 else np.ascontiguousarray(keys, dtype=np.uint32)
+    if not 0 <= int(row_begin) <= int(row_end) <= eptr.size - 1:
+        raise ValueError("row range outside eptr")
+    n = int(threads) if threads > 0 else min(16, os.cpu_count() or 1)
+    h = lib.grg_host_build_panels(int(row_begin), int(row_end), eptr.ctypes.data, keys.ctypes.data, int(panel_rows),
+                                  int(max_staged), int(heavy_threshold), n)
+    if not h:
+        raise RuntimeError(f"grg_host_build_panels failed: {lib.grg_host_last_error().decode()}")
+    try:
+        sz = np.zeros(6, dtype=np.int64)
+        if lib.grg_host_panels_sizes(h, sz.ctypes.data) != 0:
+            raise RuntimeError(f"grg_host_panels_sizes failed: {lib.grg_host_last_error().decode()}")
+        n_pan, n_pass, n_src, n_ent, max_ent, max_passes = (int(v) for v in sz)
+        pan_pass = np.empty(n_pan + 1, dtype=np.int32)
+        pass_src = np.empty(n_pass + 1, dtype=np.int32)
+        pass_ent = np.empty(n_pass + 1, dtype=np.int32)
+        pass_rowoff = np.empty(max(n_pass * (int(panel_rows) + 1), 1), dtype=np.uint16)
+        src_list = np.zeros(n_src + PANEL_PAD, dtype=np.uint32)
+        lidx = np.zeros(n_ent + PANEL_PAD, dtype=np.uint8)
+        if lib.grg_host_panels_copy(h, pan_pass.ctypes.data, pass_src.ctypes.data, pass_ent.ctypes.data, pass_rowoff.ctypes.data,
+                                    src_list.ctypes.data, lidx.ctypes.data) != 0:
+            raise RuntimeError(f"grg_host_panels_copy failed: {lib.grg_host_last_error().decode()}")
+    finally:
+        lib.grg_host_panels_free(h)
+    return Panels(panel_rows=int(panel_rows), max_staged=int(max_staged), max_pass_entries=max_ent, max_passes=max_passes,
+                  pan_pass=pan_pass,
+                  pass_src=pass_src, pass_ent=pass_ent, pass_rowoff=pass_rowoff, src_list=src_list, lidx=lidx)

@@ -252,8 +252,13 @@ def schedule_capacity_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeReq
     return int(configs * per_config)
 
 
+PANEL_ROWS = 64       # rows of one panel of the panel launches
+PANEL_STAGED = 128    # source rows staged per panel pass
+
+
 def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequirements) -> int:
-    """Upper bound of the tile tables of the tiled level launches.
+    """Upper bound of the tile tables of the tiled level launches and of the panels
+    of the panel launches.
 
     Per sweep configuration (up; down; down with missing-data input): a uint16
     staged index per gather entry (plus 16 bytes of padding), at most one uint32
@@ -263,6 +268,7 @@ def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequiremen
     most one tile per row plus one per level).
     """
     N, L, E = scan.num_nodes, scan.num_levels, scan.num_edges
+    panels = N // PANEL_ROWS + L + 1
     entries = []  # (entries, merged keys kept: more than one segment)
     if pair.plan_up is not None:
         entries.append((E, False))
@@ -270,7 +276,16 @@ def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequiremen
         entries.append((E + scan.selector_mut_nnz, True))
         if req.need_down_miss_input:
             entries.append((E + scan.selector_mut_nnz + scan.selector_miss_nnz, True))
-    return int(sum(6 * e + (4 * e if keyed else 0) + 16 + 4 * (N + 1) + 8 * (N + L + 1) for e, keyed in entries))
+    tiles = sum(6 * e + (4 * e if keyed else 0) + 16 + 4 * (N + 1) + 8 * (N + L + 1) for e, keyed in entries)
+    # panels (no more than one per PANEL_ROWS rows plus one per level): a uint8 entry
+    # position and at most one uint32 staged source per entry, and per pass (at most
+    # one per PANEL_STAGED sources plus one per panel) two int32 offsets and
+    # PANEL_ROWS + 1 uint16 row offsets
+    pan = 0
+    for e, _ in entries:
+        passes = e // PANEL_STAGED + panels
+        pan += 5 * e + 80 + 4 * (panels + 1) + (8 + 2 * (PANEL_ROWS + 1)) * (passes + 1)
+    return int(tiles + pan)
 
 
 def plan_hip_layout(
@@ -434,4 +449,4 @@ def plan_hip_layout(
     )
 
 
-__all__ = ["HEAVY_THRESHOLD", "SPLIT_CHUNK", "SPLIT_MIN", "split_items_bound", "HipArtifactLayout", "HipBlockPlan", "HipLayout", "HipSlotPlan", "enabled_max_k", "plan_hip_layout", "side_io_rows", "tile_bytes_for"]
+__all__ = ["HEAVY_THRESHOLD", "PANEL_ROWS", "PANEL_STAGED", "SPLIT_CHUNK", "SPLIT_MIN", "split_items_bound", "HipArtifactLayout", "HipBlockPlan", "HipLayout", "HipSlotPlan", "enabled_max_k", "plan_hip_layout", "side_io_rows", "tile_bytes_for"]

@@ -42,7 +42,7 @@ import scipy.sparse as sp
 
 from pygrgl_spmv.backends.base import copy_struct_checked, relink_stream_dependencies
 from pygrgl_spmv.backends.hip import ffi
-from pygrgl_spmv.backends.hip.layout import SPLIT_CHUNK, SPLIT_MIN, HipLayout, enabled_max_k, side_io_rows
+from pygrgl_spmv.backends.hip.layout import PANEL_ROWS, PANEL_STAGED, SPLIT_CHUNK, SPLIT_MIN, HipLayout, enabled_max_k, side_io_rows
 from pygrgl_spmv.backends.types import Direction, InitMode
 from pygrgl_spmv.grg import BoundGRG, _CudaMatmulSpec
 from pygrgl_spmv.grg.artifact import _load_grg_spmv_host, iter_artifact_blocks
@@ -71,6 +71,10 @@ KERNEL_UNROLL = 8  # gathers in flight per lane and round in the kernel
 TILE_MAX_STAGED = 128   # staged source rows of one tile (LDS rows)
 TILE_MAX_ROWS = 256     # rows of one tile
 TILE_MAX_ENTRIES = 2048  # gather entries of one tile
+PANEL_MIN_MEAN = 32.0   # levels whose light rows average fewer entries get no panels
+PANEL_MIN_ROWS = 1024   # ... nor levels with fewer rows
+PANEL_MAX_PASS_BYTES = 16384  # entry positions of one pass (with 32 bytes of slack) the panel kernel stages
+PANEL_MAX_PASSES = 2048       # passes of one panel (their entry offsets sit in LDS)
 DEGREE_WINDOW = 4096    # nodes per degree-sorted run of the device node order
 DEGREE_MIN_MEAN = 6.0   # levels with a lower mean up-sweep degree keep the plain locality order
 
@@ -111,6 +115,21 @@ def tile_lds_bytes(dtype_code: int, cpl: int, kl: int, staged_rows: int, max_row
         return (int(b) + 15) // 16 * 16
 
     return r16(int(staged_rows) * (kl * cpl + pad) * elem) + r16((int(max_rows) + 1) * 4) + r16((int(max_entries) + 16) * 2) + 8 * 4 * 64
+
+
+def panel_lds_bytes(dtype_code: int, cpl: int, kl: int, staged_rows: int, panel_rows: int, max_entries: int,
+                    max_passes: int) -> int:
+    """LDS of one panel workgroup: the panel's int32 pass entry offsets, one pass's
+    staged rows (as in a tile), its uint16 row offsets and uint8 entry positions with
+    32 bytes of slack (each part rounded to 16 bytes), and the 2 KiB reduction buffer."""
+    elem = 8 if dtype_code == ffi.F64 else 4
+    pad = cpl if cpl * elem >= 16 else 16 // elem
+
+    def r16(b):
+        return (int(b) + 15) // 16 * 16
+
+    return (r16((int(max_passes) + 1) * 4) + r16(int(staged_rows) * (kl * cpl + pad) * elem) + r16((int(panel_rows) + 1) * 2)
+            + r16(int(max_entries) + 32) + 8 * 4 * 64)
 
 
 @dataclass(frozen=True)
@@ -200,6 +219,7 @@ class _HipArtifact:
     down_events: list = field(default_factory=list)
     max_items: int = 0                                # largest split-item count of one level launch
     tiles: dict = field(default_factory=dict)         # config -> per-level tile offsets and LDS maxima
+    panels: dict = field(default_factory=dict)        # (config, level) -> (panels, most entries of a pass, most passes of a panel)
 
 
 @dataclass
@@ -393,7 +413,7 @@ class HipBatchedSweep:
                     for i in range(n):
                         a = ffi.GrgHipGatherArgs()
                         ctypes.pointer(a)[0] = arr[i]
-                        a.flags &= ~ffi.FLAG_TILED
+                        a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_PANEL)
                         launches.append(a)
                         phase_of.append(p)
                         p += 1
@@ -447,7 +467,8 @@ class HipRuntime:
                  split_chunk: int = SPLIT_CHUNK, fixed_shape: dict | None = None, tiles: bool = True,
                  tile_max_staged: int = TILE_MAX_STAGED, tile_max_rows: int = TILE_MAX_ROWS,
                  tile_max_entries: int = TILE_MAX_ENTRIES, degree_window: int = DEGREE_WINDOW,
-                 degree_min_mean: float = DEGREE_MIN_MEAN) -> None:
+                 degree_min_mean: float = DEGREE_MIN_MEAN, panels: bool = False, panel_min_mean: float = PANEL_MIN_MEAN,
+                 panel_min_rows: int = PANEL_MIN_ROWS) -> None:
         """``autotune``: time the admissible launch shapes of every level launch when an
         op is prepared (otherwise a fixed default shape). ``split_min`` /
         ``split_chunk``: rows of gather degree above ``split_min`` are summed in
@@ -463,7 +484,11 @@ class HipRuntime:
         device node order sorts each run of ``degree_window`` consecutive nodes of a
         level (levels whose mean gather degree in the up sweep is at least
         ``degree_min_mean``) by degree, so the rows a wave sums in lockstep have
-        similar lengths (0 keeps the plain locality order)."""
+        similar lengths (0 keeps the plain locality order). ``panels`` (off by default:
+        on MI355X the per-row forms are faster at every level measured so far): with the
+        tile tables, also build the panels of the panel launches (levels whose light rows
+        average at least ``panel_min_mean`` entries, with at least ``panel_min_rows``
+        rows) and let the tuner choose them."""
         if int(degree_window) < 0:
             raise ValueError("degree_window must be >= 0")
         if int(split_min) < 1 or int(split_chunk) < 1:
@@ -488,6 +513,9 @@ class HipRuntime:
         self.tile_max_staged = int(tile_max_staged)
         self.tile_max_rows = int(tile_max_rows)
         self.tile_max_entries = int(tile_max_entries)
+        self.use_panels = bool(panels)
+        self.panel_min_mean = float(panel_min_mean)
+        self.panel_min_rows = int(panel_min_rows)
         self.degree_window = int(degree_window)
         self.degree_min_mean = float(degree_min_mean)
         self._dtype_code = ffi.F64 if np.dtype(layout.dtype) == np.float64 else ffi.F32C
@@ -860,6 +888,10 @@ class HipRuntime:
                         D[f"{name}_ekey"] = self._to_dev(merged_keys(t.eptr, segs_h).view(np.int32))
                     art.tiles[name] = (t.level_tile_off.copy(), t.level_max_staged.copy(), t.level_max_rows.copy(),
                                        t.level_max_entries.copy())
+                    if self.use_panels:
+                        keys = merged_keys(t.eptr, segs_h) if len(segs_h) > 1 else segs_h[0][1]
+                        self._build_panels(art, name, t.eptr, keys, off)
+                        del keys
                     del t
                 del seg_host
             # streamed ops with slot-reuse links, and their events
@@ -873,6 +905,43 @@ class HipRuntime:
 
     def _elem(self) -> int:
         return 8 if self._dtype_code == ffi.F64 else 4
+
+    def _build_panels(self, art: _HipArtifact, cfg: str, eptr: np.ndarray, keys: np.ndarray, off: np.ndarray) -> None:
+        """Panels of the level launches of sweep configuration ``cfg`` whose light rows
+        average at least ``panel_min_mean`` entries (``grg_host_build_panels``)."""
+        from pygrgl_spmv.backends.hip.host import build_panels
+
+        D = art.dev
+        for l in range(off.size - 1):
+            lo, hi = int(off[l]), int(off[l + 1])
+            sched = art.sched.get((cfg, l))
+            if sched is None or hi - lo < max(1, self.panel_min_rows) or sched.mean_degree < self.panel_min_mean:
+                continue
+            pn = build_panels(eptr, keys, lo, hi, panel_rows=PANEL_ROWS, max_staged=PANEL_STAGED,
+                              heavy_threshold=int(sched.heavy_threshold))
+            if pn.max_pass_entries + 32 > PANEL_MAX_PASS_BYTES or pn.max_passes > PANEL_MAX_PASSES:
+                continue
+            tag = f"{cfg}_pan{l}"
+            D[f"{tag}_pass"] = self._to_dev(pn.pan_pass)
+            D[f"{tag}_src_off"] = self._to_dev(pn.pass_src)
+            D[f"{tag}_ent_off"] = self._to_dev(pn.pass_ent)
+            D[f"{tag}_rowoff"] = self._to_dev(pn.pass_rowoff.view(np.int16))
+            D[f"{tag}_src"] = self._to_dev(pn.src_list.view(np.int32))
+            D[f"{tag}_lidx"] = self._to_dev(pn.lidx.view(np.int8))
+            art.panels[(cfg, l)] = (pn.num_panels, pn.max_pass_entries, pn.max_passes)
+
+    def _panel_fields(self, art: _HipArtifact, cfg: str, level: int):
+        """Panel fields of one level launch of sweep configuration ``cfg``, or None."""
+        meta = art.panels.get((cfg, level))
+        if meta is None:
+            return None
+        D = art.dev
+        tag = f"{cfg}_pan{level}"
+        return {"pan_pass": D[f"{tag}_pass"].data_ptr(), "pass_src": D[f"{tag}_src_off"].data_ptr(),
+                "pass_ent": D[f"{tag}_ent_off"].data_ptr(), "pass_rowoff": D[f"{tag}_rowoff"].data_ptr(),
+                "pan_src": D[f"{tag}_src"].data_ptr(), "pan_lidx": D[f"{tag}_lidx"].data_ptr(), "num_panels": int(meta[0]),
+                "panel_rows": PANEL_ROWS, "panel_staged": PANEL_STAGED, "panel_max_entries": int(meta[1]),
+                "panel_max_passes": int(meta[2])}
 
     def _tile_fields(self, art: _HipArtifact, cfg: str, level: int):
         """Tile fields of one level launch of sweep configuration ``cfg``, or None."""
@@ -892,9 +961,10 @@ class HipRuntime:
 
     def _args(self, *, rows, k, out, ld_out, segs=(), maps=(), row_scale=None, col_vec=None, accumulate=False,
               sched: _LevelSchedule | None = None, sched_base: int = 0, mean_degree: float = 1.0,
-              out_base=0, tiles: dict | None = None, partials=None) -> ffi.GrgHipGatherArgs:
+              out_base=0, tiles: dict | None = None, panels: dict | None = None, partials=None) -> ffi.GrgHipGatherArgs:
         """Launch descriptor for rows [rows); ``sched`` adds the level's heavy rows,
-        ``tiles`` (from ``_tile_fields``) the level's tile tables."""
+        ``tiles`` (from ``_tile_fields``) the level's tile tables, ``panels`` (from
+        ``_panel_fields``) its panels."""
         a = ffi.GrgHipGatherArgs()
         a.dtype = self._dtype_code
         a.accumulate = 1 if accumulate else 0
@@ -919,6 +989,9 @@ class HipRuntime:
         if tiles:
             for name, value in tiles.items():
                 setattr(a, name, int(value))
+        if panels:
+            for name, value in panels.items():
+                setattr(a, name, int(value))
         for name, value in self.fixed_shape.items():
             setattr(a, name, int(value))
         if (a.flags & ffi.FLAG_MERGED) and not a.ekey:
@@ -932,13 +1005,22 @@ class HipRuntime:
             else:
                 a.unroll = 4 if a.unroll == 4 else 8
                 a.rows_per_group = 1
+        if a.flags & ffi.FLAG_PANEL:
+            # a panel launch needs the level's panels and one panel row per lane group
+            group = 256 // PANEL_ROWS
+            if not panels:
+                a.flags &= ~ffi.FLAG_PANEL
+            else:
+                a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_MERGED)
+                a.kl = min(a.kl, group)
+                a.es = group // a.kl
         if a.flags & ffi.FLAG_TILED:
             if not tiles:
                 a.flags &= ~ffi.FLAG_TILED
             else:
                 while a.kl > 1 and a.kl * a.cpl * self._elem() > ffi.TILE_CHUNK_BYTES:
                     a.kl //= 2
-        if a.kl * a.es > 64:
+        if a.kl * a.es > 64 and not a.flags & ffi.FLAG_PANEL:
             a.es = 64 // a.kl
         a.out, a.ld_out, a.out_base = int(out), int(ld_out), int(out_base)
         a.nseg = len(segs)
@@ -1012,6 +1094,18 @@ class HipRuntime:
                     for es_s in sorted({es, max(1, es // 2), min(64 // kl, es * 2)}):
                         for u in (4, 8):
                             time_shape(cpl, kl, es_s, u, 1, base)
+                if a.pan_pass:
+                    # the panel form: one panel row per lane group (kl * es = 256 / panel_rows)
+                    group = 256 // a.panel_rows
+                    n_before = len(timings)
+                    for cpl, kl in column_shapes(a.dtype, a.k, lds):
+                        for klp in sorted({min(kl, group), max(1, min(kl, group) // 2)}):
+                            if a.panel_staged * klp > 1024 or panel_lds_bytes(a.dtype, cpl, klp, a.panel_staged, a.panel_rows, a.panel_max_entries, a.panel_max_passes) > TILE_LDS_LIMIT:
+                                continue
+                            time_shape(cpl, klp, group // klp, 8, 1, ffi.FLAG_PANEL)
+                    if len(timings) > n_before:
+                        _, cpl, kl, es, unroll, rows, _ = min(timings[n_before:])
+                        time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_PANEL | ffi.FLAG_XCD_REMAP)
                 if a.tile_row:
                     # the LDS-staged form of the same launch: column chunks of at most
                     # TILE_CHUNK_BYTES, every lane-group width
@@ -1106,7 +1200,8 @@ class HipRuntime:
             sched = art.sched[(cfg, l)] if segs else None
             level_args = self._args(rows=(lo, hi), k=k, out=V, ld_out=ldv, segs=segs, maps=maps,
                                     row_scale=row_scale_init, col_vec=col_vec_init, sched=sched, sched_base=sched_base,
-                                    tiles=self._tile_fields(art, cfg, l) if segs else None, partials=partials)
+                                    tiles=self._tile_fields(art, cfg, l) if segs else None,
+                                    panels=self._panel_fields(art, cfg, l) if segs else None, partials=partials)
             if segs:
                 self._tune(level_args, (art.path, cfg, l, k))
             if sched is not None and sched.num_items:

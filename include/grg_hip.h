@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 10
+#define GRG_HIP_ABI_VERSION 11
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -67,6 +67,8 @@ enum GrgHipDtype {
  * once (software pipeline); heavy rows follow over the same grid. Needs
  * GRG_HIP_FLAG_MERGED or a single segment; unroll 4 or 8; rows_per_group is 1. */
 #define GRG_HIP_FLAG_STREAM 32
+/* Panel launch: see the panel fields of GrgHipGatherArgs. */
+#define GRG_HIP_FLAG_PANEL 64
 /* Bytes of one staged row (kl * cpl columns) in a tiled launch, at most. */
 #define GRG_HIP_TILE_CHUNK_BYTES 128
 /* Staged rows of one tile, at most (uint16 LDS indices). */
@@ -166,6 +168,37 @@ typedef struct GrgHipGatherArgs {
   /* Workgroups of a streamed launch (GRG_HIP_FLAG_STREAM) along the rows, at
    * most; 0 means as many as are resident at once. */
   int64_t stream_blocks;
+  /* Panel launch (GRG_HIP_FLAG_PANEL; no reference counterpart: an LDS-staged
+   * form for rows that share many sources). Rows [row_begin, row_end) are cut
+   * into num_panels panels of panel_rows consecutive rows (the last one cut at
+   * row_end); panel_rows * kl * es = 256, so each lane group owns one row of
+   * its panel and keeps that row's sums in registers. A panel's distinct
+   * sources are staged in passes of at most panel_staged rows: pass g (panel
+   * p's passes are [pan_pass[p], pan_pass[p + 1])) copies the source rows
+   * pan_src[pass_src[g] .. pass_src[g + 1]) (each (segment << 30) | source
+   * row; one column chunk of kl * cpl columns; every pass of a panel but its
+   * last stages exactly panel_staged rows) into LDS with its entry lists,
+   * and row i of the panel adds the staged rows pan_lidx[pass_ent[g] + j] for
+   * j in [pass_rowoff[g * (panel_rows + 1) + i], pass_rowoff[g * (panel_rows
+   * + 1) + i + 1]). The next pass's sources are loaded while the current one
+   * is summed. Heavy rows (as above) hold no panel entries and are produced by
+   * the heavy workgroups after the panels: grid = num_panels + num_heavy.
+   * kl * es is at least 2 (panel_rows at most 128).
+   * panel_staged is in [1, 256] with panel_staged * kl <= 1024;
+   * panel_max_entries bounds the entries of one pass (at most 16352) and
+   * panel_max_passes the passes of one panel; pan_lidx
+   * is 16-byte aligned and readable 16 bytes past its last entry. */
+  const int32_t* pan_pass;
+  const int32_t* pass_src;
+  const int32_t* pass_ent;
+  const uint16_t* pass_rowoff;
+  const uint32_t* pan_src;
+  const uint8_t* pan_lidx;
+  int64_t num_panels;
+  int32_t panel_rows;
+  int32_t panel_staged;
+  int32_t panel_max_entries;
+  int32_t panel_max_passes; /* most passes of one panel (sizes the LDS) */
 } GrgHipGatherArgs;
 
 /* One row-gather launch (see above). Replaces one level of the reference's

@@ -78,6 +78,41 @@ int grg_host_build_tiles(int64_t num_rows, int32_t num_levels, const int64_t* le
 int grg_host_merged_keys(int64_t num_rows, int32_t nseg, const int32_t* const* seg_ptr, const int32_t* const* seg_idx,
                          const int32_t* eptr, uint32_t* ekey);
 
+/* Panels of the panel launch of libgrg_hip.so (GRG_HIP_FLAG_PANEL; see the
+ * panel fields of GrgHipGatherArgs in grg_hip.h) for rows [row_begin, row_end)
+ * of one level. No reference counterpart.
+ *
+ * Row r's entries are keys[eptr[r] .. eptr[r + 1]) (merged entry keys
+ * (segment << 30) | source row, or plain source rows of one segment), each key
+ * at most once per row. Panel p is rows row_begin + p * panel_rows + [0,
+ * panel_rows), cut at row_end. Rows of degree > heavy_threshold (when >= 0)
+ * contribute no entries (their heavy workgroups produce them). The panel's
+ * distinct keys, ascending, are staged max_staged at a time: pass q of a panel
+ * stages keys [q * max_staged, (q + 1) * max_staged) of that list, and lists
+ * for every row of the panel the entries whose key is in the pass, as 8-bit
+ * positions in the pass, ascending. A row's entries are therefore summed in
+ * ascending key order. panel_rows and max_staged are in [1, 256]; threads > 1
+ * builds panels on that many host threads.
+ *
+ * Returns an opaque handle (NULL on error, see grg_host_last_error), read with
+ * grg_host_panels_sizes / grg_host_panels_copy and released with
+ * grg_host_panels_free. sizes[6] = {panels, passes, sources, entries, most
+ * entries of one pass, most passes of one panel}. Copied arrays (sizes in
+ * brackets):
+ *   pan_pass    [panels + 1]: panel p's passes are [pan_pass[p], pan_pass[p + 1]);
+ *   pass_src    [passes + 1]: pass g stages src_list[pass_src[g] .. pass_src[g + 1]);
+ *   pass_ent    [passes + 1]: pass g's entries are lidx[pass_ent[g] .. pass_ent[g + 1]);
+ *   pass_rowoff [passes * (panel_rows + 1)]: row i of pass g holds its pass's entries
+ *               [pass_rowoff[g * (panel_rows + 1) + i], ... + i + 1]);
+ *   src_list    [sources]: staged keys;
+ *   lidx        [entries]: position of each entry's key in its pass. */
+void* grg_host_build_panels(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys, int32_t panel_rows,
+                            int32_t max_staged, int64_t heavy_threshold, int32_t threads);
+int grg_host_panels_sizes(const void* panels, int64_t* sizes);
+int grg_host_panels_copy(const void* panels, int32_t* pan_pass, int32_t* pass_src, int32_t* pass_ent, uint16_t* pass_rowoff,
+                         uint32_t* src_list, uint8_t* lidx);
+void grg_host_panels_free(void* panels);
+
 const char* grg_host_last_error(void);
 
 #ifdef __cplusplus

@@ -151,6 +151,56 @@ def test_streamed_light_path(art_a, oracle_a, dtype, unroll, es):
             assert O.normwise_rel_err(grg.matmul(x, direction), O.matmul(oracle_a, x.astype(np.float64), direction)) <= tol
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("remap", [False, True])
+@pytest.mark.parametrize("kl", [None, 1])
+def test_panel_launch(art_a, oracle_a, dtype, remap, kl):
+    """The panel form (rows of a panel share passes of LDS-staged sources) on every
+    level of both sweeps, with heavy and split rows beside it, several passes per
+    panel (few staged sources per pass at small sizes) and one or several column
+    chunks: exact for +-1 inputs, FP64 within 1e-12, FP32c within 1e-5."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+    from pygrgl_spmv.backends.hip import ffi
+
+    layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
+    shape = {"unroll": 8, "rows_per_group": 1, "flags": ffi.FLAG_PANEL | (ffi.FLAG_XCD_REMAP if remap else 0)}
+    if kl is not None:
+        shape["kl"] = kl
+    tol = FP64_TOL if dtype == np.float64 else 1e-5
+    with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16, panels=True, panel_min_mean=0.0,
+                    panel_min_rows=1) as rt:
+        (grg,) = rt.grgs
+        art = rt._artifacts[0]
+        assert art.panels, "no level got panels"
+        assert max(m[0] for m in art.panels.values()) > 1
+        rng = np.random.default_rng(4410 + int(remap) + (kl or 0))
+        for k in (2, 16):
+            for direction in ("up", "down"):
+                x = rng.choice(np.array([-1.0, 1.0]), size=(k, _cols(oracle_a, direction))).astype(dtype)
+                np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x.astype(np.float64), direction))
+        y = rng.standard_normal((5, oracle_a.num_mutations))
+        miss = rng.standard_normal((5, oracle_a.num_mutations))
+        got = grg.matmul(y.astype(dtype), "down", miss=miss.astype(dtype))
+        assert O.normwise_rel_err(got, O.matmul(oracle_a, y, "down", miss=miss)) <= tol
+        x = rng.standard_normal((8, oracle_a.num_individuals))
+        got = grg.matmul(x.astype(dtype), "up", by_individual=True, init="xtx")
+        assert O.normwise_rel_err(got, O.matmul(oracle_a, x, "up", by_individual=True, init="xtx")) <= tol
+
+
+def test_panel_launch_tuned(art_a, oracle_a):
+    """The tuner with panels on every level next to the other forms: whatever it picks
+    matches the oracle."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=16, max_k_down=16))
+    with HipRuntime(layout, panels=True, panel_min_mean=0.0, panel_min_rows=1) as rt:
+        (grg,) = rt.grgs
+        rng = np.random.default_rng(4420)
+        for direction in ("up", "down"):
+            x = rng.standard_normal((16, _cols(oracle_a, direction)))
+            _check(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
+
+
 @pytest.mark.parametrize("unroll,rows", [(4, 1), (8, 1), (16, 1), (4, 2), (8, 2), (4, 4)])
 @pytest.mark.parametrize("es", [1, 4])
 @pytest.mark.parametrize("merged", [False, True])

@@ -127,3 +127,72 @@ def test_merged_keys_follow_segment_order():
         np.testing.assert_array_equal(keys[eptr[r]:eptr[r + 1]], want)
     with pytest.raises(RuntimeError, match="eptr"):
         merged_keys((eptr + 1).astype(np.int32), segs)
+
+
+def _panel_sums(pn, values, row_begin, row_end):
+    """Row sums through the panels, pass by pass, as the panel launch computes them."""
+    R = pn.panel_rows
+    k = values[0].shape[1]
+    out = np.zeros((row_end - row_begin, k))
+    for p in range(pn.num_panels):
+        for g in range(int(pn.pan_pass[p]), int(pn.pan_pass[p + 1])):
+            keys = pn.src_list[pn.pass_src[g]:pn.pass_src[g + 1]]
+            assert 0 < keys.size <= pn.max_staged
+            staged = np.stack([values[int(kk >> 30)][int(kk & 0x3FFFFFFF)] for kk in keys])
+            off = pn.pass_rowoff[g * (R + 1):(g + 1) * (R + 1)].astype(np.int64)
+            ent = pn.lidx[pn.pass_ent[g]:pn.pass_ent[g + 1]].astype(np.int64)
+            assert off[0] == 0 and off[-1] == ent.size and np.all(np.diff(off) >= 0)
+            for i in range(R):
+                r = p * R + i
+                pos = ent[off[i]:off[i + 1]]
+                assert np.all(np.diff(pos) > 0)  # ascending, each source once
+                if r < row_end - row_begin:
+                    out[r] += staged[pos].sum(axis=0)
+                else:
+                    assert pos.size == 0
+    return out
+
+
+@pytest.mark.parametrize("panel_rows,max_staged,threads", [(64, 128, 1), (16, 7, 3), (1, 256, 2), (5, 1, 4)])
+def test_panels_reproduce_row_sums(panel_rows, max_staged, threads):
+    from pygrgl_spmv.backends.hip.host import build_panels, merged_keys
+
+    rng = np.random.default_rng(11)
+    rows = 300
+    segs = [(*_random_csr(rng, rows, 500, 40, heavy_rows=[(7, 200)]), 500), (*_random_csr(rng, rows, 60, 4), 60)]
+    deg = sum(np.diff(p.astype(np.int64)) for p, _, _ in segs)
+    eptr = np.zeros(rows + 1, dtype=np.int32)
+    np.cumsum(deg, out=eptr[1:])
+    keys = merged_keys(eptr, segs)
+    values = [rng.integers(-8, 9, size=(n, 3)).astype(np.float64) for _, _, n in segs]
+    plain = np.zeros((rows, 3))
+    for (p, i, _), v in zip(segs, values):
+        for r in range(rows):
+            plain[r] += v[i[p[r]:p[r + 1]]].sum(axis=0)
+    heavy = 100
+    for lo, hi in ((0, rows), (13, 250), (40, 40)):
+        pn = build_panels(eptr, keys, lo, hi, panel_rows=panel_rows, max_staged=max_staged, heavy_threshold=heavy,
+                          threads=threads)
+        assert pn.num_panels == -(-(hi - lo) // panel_rows)
+        assert pn.max_passes == (int(np.diff(pn.pan_pass).max()) if pn.num_panels else 0)
+        # every pass of a panel but its last stages exactly max_staged sources
+        for p in range(pn.num_panels):
+            n = np.diff(pn.pass_src[pn.pan_pass[p]:pn.pan_pass[p + 1] + 1])
+            assert np.all(n[:-1] == max_staged) and (n.size == 0 or 0 < n[-1] <= max_staged)
+        got = _panel_sums(pn, values, lo, hi)
+        want = plain[lo:hi].copy()
+        want[deg[lo:hi] > heavy] = 0  # heavy rows are left to their heavy workgroups
+        np.testing.assert_array_equal(got, want)
+        # every non-heavy entry appears exactly once
+        assert int(pn.pass_ent[-1]) == int(deg[lo:hi][deg[lo:hi] <= heavy].sum())
+
+
+def test_panels_reject_bad_arguments():
+    from pygrgl_spmv.backends.hip.host import build_panels
+
+    eptr = np.array([0, 2, 2], dtype=np.int32)
+    keys = np.array([3, 3], dtype=np.uint32)  # a row listing one source twice
+    with pytest.raises(RuntimeError, match="twice"):
+        build_panels(eptr, keys, 0, 2, panel_rows=2, max_staged=4)
+    with pytest.raises(RuntimeError, match="panel_rows"):
+        build_panels(eptr, np.array([1, 3], dtype=np.uint32), 0, 2, panel_rows=0, max_staged=4)
