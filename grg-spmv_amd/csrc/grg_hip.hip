@@ -602,6 +602,12 @@ __global__ __launch_bounds__(kBlock) void row_stream_kernel(const GrgHipGatherAr
   bool live_c = col_live && c1 >= c0 && !(thr >= 0 && c1 - c0 > thr);
   uint32_t kc[U];
   stream_keys<U>(keys, c0, c1, live_c, e, ES, kc);
+  // The previous batch's row is stored only after this batch's loads are issued:
+  // stores and loads share one in-order counter (vmcnt), so a store issued ahead
+  // of the gathers would make the wait for the gathers wait for the store too.
+  Acc pend[CPL];
+  int64_t pend_row = 0;
+  bool pend_live = false;
   while (b < n_batches) {
     // pointers of batch b + 2W, keys of batch b + W, gathers of batch b: independent loads
     const int64_t rf = rn + n_waves * rows_per_wave;
@@ -615,6 +621,7 @@ __global__ __launch_bounds__(kBlock) void row_stream_kernel(const GrgHipGatherAr
     stream_keys<U>(keys, n0, n1, live_n, e, ES, kn);
     T v[U][CPL];
     stream_gather<T, CPL, U>(kc, src0, src1, src2, ld0, ld1, ld2, v);
+    if (pend_live) store_row<T, CPL>(a, pend_row, col, pend);
     Acc acc[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) acc[c].zero();
@@ -640,7 +647,10 @@ __global__ __launch_bounds__(kBlock) void row_stream_kernel(const GrgHipGatherAr
 #pragma unroll
       for (int c = 0; c < CPL; ++c) acc[c].merge_xor(off);
     }
-    if (live_c && e == 0) store_row<T, CPL>(a, rc, col, acc);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) pend[c] = acc[c];
+    pend_row = rc;
+    pend_live = live_c && e == 0;
     b += n_waves;
     rc = rn;
     rn = rf;
@@ -652,6 +662,7 @@ __global__ __launch_bounds__(kBlock) void row_stream_kernel(const GrgHipGatherAr
 #pragma unroll
     for (int u = 0; u < U; ++u) kc[u] = kn[u];
   }
+  if (pend_live) store_row<T, CPL>(a, pend_row, col, pend);
   // heavy rows: one workgroup each, strided over the grid
   for (int64_t h = blockIdx.x; h < a.num_heavy; h += gridDim.x) {
     workgroup_row<T, CPL, U>(a, a.heavy_rows[h], h, (int)blockIdx.y, kl_shift, partial);

@@ -1048,9 +1048,11 @@ class HipRuntime:
         ev[-1].synchronize()
         return min(ev[i].elapsed_time(ev[i + 1]) for i in range(reps))
 
-    def _tune(self, a: ffi.GrgHipGatherArgs, key) -> None:
+    def _tune(self, a: ffi.GrgHipGatherArgs, key, private_out: bool = False) -> None:
         """Per-level autotuner, run when an op is prepared (never in ``op()``; the
-        timing launches write only the level's node rows). Stage 1 times every
+        timing launches write only the level's node rows, or, with ``private_out``
+        (write-out launches, whose rows are the caller-visible arena), a temporary
+        buffer of the same shape). Stage 1 times every
         admissible (cpl, kl, es) shape with 8 gathers in flight and one row per lane
         group; stage 2 tries the other (gathers in flight, rows per group) pairs and
         the XCD remap on the best shape. The fastest launch is cached per artifact,
@@ -1063,6 +1065,10 @@ class HipRuntime:
             ctypes.pointer(trial)[0] = a
             elem = 8 if a.dtype == ffi.F64 else 4
             timings = []
+            scratch = None
+            if private_out:
+                scratch = torch.empty(((a.row_end - a.out_base) * a.ld_out,), dtype=self._torch_dtype(), device=self.device)
+                trial.out = scratch.data_ptr()
 
             def time_shape(cpl, kl, es, unroll, rows, fl):
                 trial.cpl, trial.kl, trial.es, trial.unroll, trial.rows_per_group, trial.flags = cpl, kl, es, unroll, rows, fl
@@ -1121,11 +1127,12 @@ class HipRuntime:
                     if len(timings) > n_untiled:
                         _, cpl, kl, es, unroll, rows, _ = min(timings[n_untiled:])
                         time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_TILED | ffi.FLAG_XCD_REMAP)
-            best = min(timings)[1:]
+            top = min(timings)
+            best = top[1:]
             self._tune_cache[key] = best
             tiled = [t for t in timings if t[6] & ffi.FLAG_TILED]
             untiled = [t for t in timings if not t[6] & ffi.FLAG_TILED]
-            self.tune_log[key] = {"rows": int(a.row_end - a.row_begin), "best": min(timings),
+            self.tune_log[key] = {"rows": int(a.row_end - a.row_begin), "best": top,
                                   "untiled_ms": min(untiled)[0] if untiled else None,
                                   "tiled_ms": min(tiled)[0] if tiled else None, "timings": sorted(timings)}
         if best is not None:
@@ -1235,12 +1242,15 @@ class HipRuntime:
                     if "vec_up" not in D:
                         raise RuntimeError("missing HIP up vector endpoint bias")
                     rs, cv = (D["vec_up"].data_ptr(), 0), io1
-            batch.append(self._args(rows=(0, st.num_mutations), k=k, out=io0, ld_out=max_k,
-                                    segs=[(D["sel_ptr"].data_ptr(), D["sel_idx"].data_ptr(), V, ldv, 0)],
-                                    row_scale=rs, col_vec=cv))
+            wr = self._args(rows=(0, st.num_mutations), k=k, out=io0, ld_out=max_k,
+                            segs=[(D["sel_ptr"].data_ptr(), D["sel_idx"].data_ptr(), V, ldv, 0)], row_scale=rs, col_vec=cv)
+            self._tune(wr, (art.path, "write_sel", -1, k), private_out=True)
+            batch.append(wr)
             if spec.use_miss:
-                batch.append(self._args(rows=(0, st.num_mutations), k=k, out=io1, ld_out=max_k,
-                                        segs=[(D["miss_ptr"].data_ptr(), D["miss_idx"].data_ptr(), V, ldv, 0)]))
+                wr = self._args(rows=(0, st.num_mutations), k=k, out=io1, ld_out=max_k,
+                                segs=[(D["miss_ptr"].data_ptr(), D["miss_idx"].data_ptr(), V, ldv, 0)])
+                self._tune(wr, (art.path, "write_miss", -1, k), private_out=True)
+                batch.append(wr)
         else:
             rs = cv = None
             suffix = "_ind" if spec.by_individual else ""
@@ -1251,9 +1261,11 @@ class HipRuntime:
                 rs = (D[key].data_ptr(), 0)
                 cv = io1 if spec.init_mode == InitMode.VECTOR else None
             if spec.by_individual:
-                batch.append(self._args(rows=(0, st.num_individuals), k=k, out=io0, ld_out=max_k,
-                                        segs=[(D["ind_ptr"].data_ptr(), D["ind_idx"].data_ptr(), V, ldv, 0)],
-                                        row_scale=rs, col_vec=cv, mean_degree=float(st.ploidy)))
+                wr = self._args(rows=(0, st.num_individuals), k=k, out=io0, ld_out=max_k,
+                                segs=[(D["ind_ptr"].data_ptr(), D["ind_idx"].data_ptr(), V, ldv, 0)],
+                                row_scale=rs, col_vec=cv, mean_degree=float(st.ploidy))
+                self._tune(wr, (art.path, "write_ind", -1, k), private_out=True)
+                batch.append(wr)
             else:
                 batch.append(self._args(rows=(0, st.num_samples), k=k, out=io0, ld_out=max_k,
                                         maps=[(D["out_map"].data_ptr(), V, ldv, 0, st.num_samples)], row_scale=rs, col_vec=cv))
