@@ -191,6 +191,61 @@ def cpu_baseline(artifact: Path, stats: dict, *, k: int, use_miss: bool, budget_
     }
 
 
+def gather_ceiling(k: int = 16, rows: int = 4_000_000, degree: int = 8, reps: int = 5) -> list:
+    """Throughput of the sweep kernel (``row_gather_kernel``) on synthetic single-level
+    launches: ``rows`` output rows of ``degree`` entries each gather FP64 k-wide source
+    rows, from a source array of 64 MB or 3.3 GB, with indices either uniformly random
+    or confined to a 65,536-row window that moves with the output row. No GRG structure:
+    this is the gather rate the hardware gives this kernel, the ceiling the sweep levels
+    are compared with."""
+    import torch
+
+    from pygrgl_spmv.backends.hip import ffi
+
+    lib = ffi.hip_library()
+    dev = torch.device("cuda", 0)
+    out_rows = []
+    ptr = torch.arange(0, (rows + 1) * degree, degree, dtype=torch.int32, device=dev)
+    out = torch.empty((rows, k), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2026)
+    for src_rows in (500_000, 26_000_000):
+        src = torch.ones((src_rows, k), dtype=torch.float64, device=dev)
+        for pattern in ("random", "window"):
+            if pattern == "random":
+                idx = torch.randint(0, src_rows, (rows * degree,), dtype=torch.int64, device=dev, generator=gen)
+            else:
+                base = torch.arange(rows * degree, device=dev, dtype=torch.int64) // degree * src_rows // rows
+                idx = (base + torch.randint(0, 65536, (rows * degree,), device=dev, generator=gen)) % src_rows
+            idx = idx.to(torch.int32)
+            for unroll, es in ((8, 1), (16, 1)):
+                a = ffi.GrgHipGatherArgs()
+                a.dtype, a.accumulate, a.row_begin, a.row_end, a.k = ffi.F64, 0, 0, rows, k
+                a.cpl, a.kl, a.es, a.unroll, a.rows_per_group, a.flags = 2, k // 2, es, unroll, 1, 0
+                a.out, a.ld_out, a.out_base = out.data_ptr(), k, 0
+                a.nseg, a.nmap = 1, 0
+                a.seg[0].ptr, a.seg[0].idx, a.seg[0].src, a.seg[0].ld, a.seg[0].ptr_base = ptr.data_ptr(), idx.data_ptr(), src.data_ptr(), k, 0
+                a.heavy_threshold, a.heavy_rows, a.num_heavy = -1, 0, 0
+                lib.gather(a, stream.cuda_stream)
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+                ev[0].record(stream)
+                for i in range(reps):
+                    lib.gather(a, stream.cuda_stream)
+                    ev[i + 1].record(stream)
+                ev[-1].synchronize()
+                ms = min(ev[i].elapsed_time(ev[i + 1]) for i in range(reps))
+                gathers = rows * degree
+                rec = {"source_MB": src_rows * k * 8 / 1e6, "pattern": pattern, "unroll": unroll, "ms": ms,
+                       "G_gathers_per_s": gathers / ms / 1e6, "gathered_TB_per_s": gathers * k * 8 / ms / 1e9,
+                       "written_TB_per_s": rows * k * 8 / ms / 1e9}
+                log(f"gather ceiling: {rec}")
+                out_rows.append(rec)
+            del idx
+        del src
+    return out_rows
+
+
 def load_traffic(workload: str, k: int):
     """HBM bytes per sweep measured by rocprofv3 PMC runs of this benchmark (FETCH_SIZE and
     WRITE_SIZE over the sweep's dispatches, scaled by a plain copy), if recorded."""
@@ -218,7 +273,12 @@ def main() -> int:
                     help="after the timed region, run one device copy of this many MiB (for PMC scaling)")
     ap.add_argument("--tune-log", default="",
                     help="write the per-level tuner choices and launch times of each form to this JSON file")
+    ap.add_argument("--gather-ceiling", action="store_true",
+                    help="instead of the sweep: the sweep kernel's gather rate on synthetic random and windowed indices")
     args = ap.parse_args()
+    if args.gather_ceiling:
+        print(json.dumps({"gather_ceiling": gather_ceiling(k=int(args.k))}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
         return 2
