@@ -1082,6 +1082,50 @@ __global__ __launch_bounds__(kBlock) void panel_gather_kernel(const GrgHipGather
 }
 
 
+// ---- row-weighted sums ------------------------------------------------------
+
+// out[j] = sum_m x[j + m * ld] * w[m] (j < rows, m < cols), in two fixed-order
+// passes: workgroup b sums the flattened (m, j) elements b*256*iters + ... of a
+// grid-stride walk whose stride is a multiple of rows, so each thread keeps one
+// j; the workgroup's threads of equal j are summed in thread order into
+// partials[b * rows + j]; the second pass sums the partials of each j in
+// workgroup order. FP64 throughout the sums, so the result is deterministic.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void row_weighted_partials_kernel(const T* __restrict__ x, const T* __restrict__ w,
+                                                                      int64_t rows, int64_t cols, int64_t ld,
+                                                                      double* __restrict__ partials) {
+  __shared__ double part[kBlock];
+  const int64_t per = (kBlock / rows) * rows;  // threads in use: a multiple of rows
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  if (t < per) {
+    const int64_t j = t % rows;
+    const int64_t stride = (int64_t)gridDim.x * per;
+    const int64_t total = rows * cols;
+    for (int64_t f = (int64_t)blockIdx.x * per + t; f < total; f += stride) {
+      const int64_t m = f / rows;
+      acc += (double)x[j + m * ld] * (double)w[m];
+    }
+  }
+  part[t] = acc;
+  __syncthreads();
+  if (t < rows) {
+    double s = 0.0;
+    for (int64_t i = t; i < per; i += rows) s += part[i];
+    partials[(int64_t)blockIdx.x * rows + t] = s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void row_weighted_finish_kernel(const double* __restrict__ partials, int64_t rows,
+                                                                    int32_t n_blocks, T* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= rows) return;
+  double s = 0.0;
+  for (int32_t b = 0; b < n_blocks; ++b) s += partials[(int64_t)b * rows + j];
+  out[j] = (T)s;
+}
+
 // ---- job tables -------------------------------------------------------------
 
 constexpr uint32_t kSpinLimit = 1u << 24;  // barrier polls before a wait is declared failed
@@ -1595,6 +1639,32 @@ int grg_hip_gather_batch(const GrgHipGatherArgs* args, int32_t n, void* stream) 
       return rc;
     }
   }
+  return 0;
+}
+
+int grg_hip_row_weighted_sum(int32_t dtype, const void* x, int64_t rows, int64_t cols, int64_t ld, const void* w, void* out,
+                             double* partials, int32_t n_blocks, void* stream) {
+  g_last_error.clear();
+  if (dtype != GRG_HIP_F64 && dtype != GRG_HIP_F32C) return fail("unknown dtype code " + std::to_string(dtype));
+  if (rows < 1 || rows > kBlock) return fail("rows must be in [1, 256]");
+  if (cols < 0 || ld < rows) return fail("cols must be >= 0 and ld >= rows");
+  if (n_blocks < 1) return fail("n_blocks must be >= 1");
+  if (!out || !partials || (cols > 0 && (!x || !w))) return fail("null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 fin((unsigned)((rows + kBlock - 1) / kBlock));
+  if (dtype == GRG_HIP_F64) {
+    hipLaunchKernelGGL(row_weighted_partials_kernel<double>, dim3(n_blocks), dim3(kBlock), 0, s, static_cast<const double*>(x),
+                       static_cast<const double*>(w), rows, cols, ld, partials);
+    hipLaunchKernelGGL(row_weighted_finish_kernel<double>, fin, dim3(kBlock), 0, s, partials, rows, n_blocks,
+                       static_cast<double*>(out));
+  } else {
+    hipLaunchKernelGGL(row_weighted_partials_kernel<float>, dim3(n_blocks), dim3(kBlock), 0, s, static_cast<const float*>(x),
+                       static_cast<const float*>(w), rows, cols, ld, partials);
+    hipLaunchKernelGGL(row_weighted_finish_kernel<float>, fin, dim3(kBlock), 0, s, partials, rows, n_blocks,
+                       static_cast<float*>(out));
+  }
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return fail(std::string("kernel launch failed: ") + hipGetErrorString(err), 2);
   return 0;
 }
 

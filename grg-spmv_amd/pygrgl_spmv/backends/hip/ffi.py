@@ -128,6 +128,9 @@ class HipLibrary:
         lib.grg_hip_jobs_run.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32,
                                          c_void_p, c_void_p, c_void_p]
         lib.grg_hip_jobs_run.restype = ctypes.c_int
+        lib.grg_hip_row_weighted_sum.argtypes = [c_int32, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                                                 c_int32, c_void_p]
+        lib.grg_hip_row_weighted_sum.restype = ctypes.c_int
         version = int(lib.grg_hip_abi_version())
         if version != ABI_VERSION:
             raise RuntimeError(f"libgrg_hip.so ABI version {version} != expected {ABI_VERSION}; rebuild the extension")
@@ -172,6 +175,12 @@ class HipLibrary:
                                               c_void_p(blocks_host.ctypes.data), int(dtype), int(nphase), int(bool(persistent)),
                                               int(grid), c_void_p(int(barrier_dev)), c_void_p(int(status_dev)),
                                               c_void_p(int(stream_ptr))), "grg_hip_jobs_run")
+
+    def row_weighted_sum(self, dtype: int, x_ptr: int, rows: int, cols: int, ld: int, w_ptr: int, out_ptr: int,
+                         partials_ptr: int, n_blocks: int, stream_ptr: int) -> None:
+        self._check(self.lib.grg_hip_row_weighted_sum(int(dtype), c_void_p(int(x_ptr)), int(rows), int(cols), int(ld),
+                                                      c_void_p(int(w_ptr)), c_void_p(int(out_ptr)), c_void_p(int(partials_ptr)),
+                                                      int(n_blocks), c_void_p(int(stream_ptr))), "grg_hip_row_weighted_sum")
 
     def device_count(self) -> int:
         out = c_int32(0)

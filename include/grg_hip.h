@@ -257,6 +257,15 @@ int grg_hip_jobs_run(const GrgHipJob* jobs, const int32_t* phase_job_off, const 
                      int32_t grid, uint32_t* barrier, int32_t* status, void* stream);
 
 /* Thread-local description of the last failure ("" if none). */
+/* Row-weighted sums for the standardized X^T X operator: out[j] = sum_m
+ * x[j + m * ld] * w[m] for j < rows (at most 256), m < cols, with x, w and out
+ * in the value type of dtype (FP32 for GRG_HIP_F32C) and FP64 sums in a fixed
+ * order (deterministic). partials holds n_blocks * rows doubles of scratch
+ * owned by the caller. Replaces the reference's CuPy RawKernel
+ * row_weighted_sum (scripts/pca_bench/__main__.py:32-61). */
+int grg_hip_row_weighted_sum(int32_t dtype, const void* x, int64_t rows, int64_t cols, int64_t ld, const void* w, void* out,
+                             double* partials, int32_t n_blocks, void* stream);
+
 const char* grg_hip_last_error(void);
 
 /* GRG_HIP_ABI_VERSION of the loaded library. */
