@@ -50,6 +50,8 @@ class StandardizedGrgOperator:
         tdt = runtime._torch_dtype()
         self._ops = []
         self.means, self.scales = [], []
+        self.m_per_grg = []
+        self._inv_scales = []
         self.m_proj = 0
         for grg in grgs:
             ones = np.ones((1, grg.num_samples))
@@ -64,7 +66,10 @@ class StandardizedGrgOperator:
             self._ops.append((up, down))
             self.means.append(mean)
             self.scales.append(scale)
-            self.m_proj += int(np.count_nonzero(scale > 0.0))
+            inv = np.where(scale > 0.0, 1.0 / np.where(scale > 0.0, scale, 1.0), 0.0)
+            self._inv_scales.append(torch.from_numpy(inv).to(self.device))
+            self.m_per_grg.append(int(np.count_nonzero(scale > 0.0)))
+            self.m_proj += self.m_per_grg[-1]
         if self.m_proj == 0:
             raise ValueError("no polymorphic variant: the operator is empty")
         # sum over model variants of ||P X_s[:, j]||^2 (= n - 1 each, core.py:950-954)
@@ -93,6 +98,37 @@ class StandardizedGrgOperator:
             out += down.output.T.to(torch.float64)
         out /= float(self.m_proj)
         return center(out)[:, :c]
+
+    def apply_parts(self, V):
+        """Per GRG g, X_g,s X_g,s^T P V (n x c FP64, neither divided by M nor centred):
+        the terms whose sum over the included GRGs gives ``apply_k`` with a GRG left out
+        (``apply_k(..., exclude_chrom=...)``, reference ``core.py:1282-1301``)."""
+        import torch
+
+        c = int(V.shape[1])
+        W = self._pad(center(V.to(dtype=torch.float64)))
+        parts = []
+        for up, down in self._ops:
+            up.input.copy_(W.T)
+            up()
+            down()
+            parts.append(down.output.T[:, :c].to(torch.float64).clone())
+        return parts
+
+    def scores(self, V):
+        """Per GRG g, X_g,s^T P V (M_g x c FP64; 0 on monomorphic variants), reference
+        ``scores`` ``core.py:1241-1252``: one up sweep, whose write-out applies scale^2,
+        then a division by the scale in FP64."""
+        import torch
+
+        c = int(V.shape[1])
+        W = self._pad(center(V.to(dtype=torch.float64)))
+        out = []
+        for (up, _), inv in zip(self._ops, self._inv_scales):
+            up.input.copy_(W.T)
+            up()
+            out.append(up.output.T[:, :c].to(torch.float64) * inv[:, None])
+        return out
 
     def genetic_values(self, betas):
         """P X_s beta for per-GRG effect vectors ``betas[g]`` (M_g x c numpy), as n x c FP64."""

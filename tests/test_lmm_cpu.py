@@ -109,3 +109,122 @@ def test_mlp_library_exports():
     with pytest.raises(RuntimeError, match="HIP grg_mlp_adam failed: status 1: Adam step"):
         m.check(m.lib.grg_mlp_adam(ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8), 1,
                                    0.1, 0.9, 0.999, 1e-8, 0, None), "grg_mlp_adam")
+
+
+# ---- variance components and LOCO (oracle pins and host logic) ----
+
+@pytest.fixture(scope="module")
+def two_grgs(tmp_path_factory):
+    from pygrgl_spmv.synth import SynthConfig, write_artifact
+
+    d = tmp_path_factory.mktemp("vc")
+    out = []
+    for seed, m in ((81, 600), (82, 400)):
+        cfg = SynthConfig(num_samples=100, num_mutations=m, seed=seed, ploidy=2, num_trees=50, keep_fraction=0.35,
+                          window_fraction=0.1, num_segments=2)
+        out.append(L.dosages(O.load_artifact(write_artifact(cfg, d / f"g{seed}.grg_spmv"))))
+    return out
+
+
+def test_score_square_sums_equal_m_ztkz(two_grgs):
+    std = L.standardized(two_grgs)
+    Xs = [x for x, _ in std]
+    K = L.kinship_of(Xs)
+    m = sum(x.shape[1] for x in Xs)
+    Z = np.random.default_rng(1).standard_normal((K.shape[0], 3))
+    PZ = L.project(Z)
+    np.testing.assert_allclose(L.score_square_sums(Xs, Z), m * (PZ * (K @ PZ)).sum(axis=0), rtol=1e-10)
+    np.testing.assert_allclose(K, L.dense_kinship(two_grgs)[0], atol=1e-12)
+
+
+def test_mc_scaling_oracle_is_zero_for_isotropic_kinship():
+    """K = c P (orthonormal centred columns): beta / eps = M c for every column, so f_REML = 0."""
+    rng = np.random.default_rng(2)
+    n, m = 40, 39
+    Q, _ = np.linalg.qr(L.project(rng.standard_normal((n, m))))
+    Xs = [Q[:, :20] * 3.0, Q[:, 20:] * 3.0]
+    G, E, y = rng.standard_normal((n, 4)), rng.standard_normal((n, 4)), rng.standard_normal(n)
+    for ld in (-1.0, 0.3, 2.0):
+        r = L.mc_scaling(Xs, G, E, y, ld)
+        assert abs(r["f_reml"]) < 1e-10
+        np.testing.assert_allclose(r["f_jacks"], 0.0, atol=1e-10)
+
+
+def test_mc_scaling_oracle_data_equal_to_a_trial(two_grgs):
+    Xs = [x for x, _ in L.standardized(two_grgs)]
+    n = Xs[0].shape[0]
+    rng = np.random.default_rng(3)
+    G, E = rng.standard_normal((n, 5)), rng.standard_normal((n, 5))
+    ld = 0.4
+    y = E[:, 2] * np.sqrt(np.exp(ld)) + G[:, 2]
+    r = L.mc_scaling(Xs, G, E, y, ld)
+    assert r["f_reml"] == pytest.approx(r["f_rands_as_data"][2], abs=1e-10)
+    H = L.kinship_of(Xs) + np.exp(ld) * np.eye(n)
+    np.testing.assert_allclose(H @ r["hinv_y"], L.project(y.reshape(n, 1))[:, 0], atol=1e-9)
+
+
+def test_loco_oracle_two_grgs_is_the_other_kinship(two_grgs):
+    std = L.standardized(two_grgs)
+    Xs = [x for x, _ in std]
+    n = Xs[0].shape[0]
+    y = np.random.default_rng(4).standard_normal(n)
+    Z = L.loco_solutions(Xs, y, 0.9)
+    for c in range(2):
+        K_other, _ = L.dense_kinship([two_grgs[1 - c]])
+        np.testing.assert_allclose(Z[:, c], L.project(np.linalg.solve(K_other + 0.9 * np.eye(n), L.project(y[:, None])))[:, 0],
+                                   atol=1e-10)
+
+
+def test_reml_objective_known_values():
+    from pygrgl_spmv.lmm.variance import reml_objective
+
+    f, jacks, as_data = reml_objective(2.0, 1.0, [1.0, 3.0], [1.0, 3.0])
+    assert f == pytest.approx(np.log(2.0))
+    assert jacks == pytest.approx((np.log(2.0), np.log(2.0), np.log(2.0)))
+    assert as_data == pytest.approx((0.0, 0.0))
+    f, jacks, _ = reml_objective(1.0, 1.0, [1.0, 4.0], [2.0, 2.0])   # sum beta / sum eps = 5/4
+    assert f == pytest.approx(np.log(0.8))
+    assert jacks[0] == pytest.approx(np.log(1.0 / 2.0)) and jacks[1] == pytest.approx(np.log(2.0))
+    with pytest.raises(RuntimeError, match="invalid MC-scaling"):
+        reml_objective(1.0, 0.0, [1.0], [1.0])
+
+
+class _Eval:
+    def __init__(self, f):
+        self.f, self.calls = f, []
+
+    def __call__(self, ld):
+        from types import SimpleNamespace
+
+        self.calls.append(ld)
+        return SimpleNamespace(log_delta=ld, f_reml=self.f(ld))
+
+
+def test_secant_search_finds_a_linear_root_and_clips():
+    from pygrgl_spmv.lmm.variance import secant_search
+
+    at = {0.25: 1.0, 0.125: 2.0, 0.5: 0.0}.__getitem__       # log delta falls as h2 rises
+    ev = _Eval(lambda ld: 0.7 * (1.6 - ld))                   # f(1) > 0 -> second start at h2 = 0.5
+    best, tried = secant_search(ev, at)
+    assert ev.calls[:2] == [1.0, 0.0] and best.log_delta == pytest.approx(1.6, abs=1e-12)
+    assert 3 <= len(tried) <= 7                                # two starts, at most five secant steps
+    ev = _Eval(lambda ld: 0.1 - 0.01 * ld)                     # root at 10: start at h2 = 0.5 then secant
+    best, _ = secant_search(ev, at)
+    assert best.log_delta == pytest.approx(10.0)
+    ev = _Eval(lambda ld: 0.1 - 0.001 * ld)                    # root at 100: clipped to 10
+    best, _ = secant_search(ev, at)
+    assert max(ev.calls) == 10.0 and best.log_delta == 10.0
+    ev = _Eval(lambda ld: ld - 1.3)                            # f(1) < 0 -> second start at h2 = 0.125
+    best, _ = secant_search(ev, at)
+    assert ev.calls[1] == 2.0 and best.log_delta == pytest.approx(1.3)
+
+
+def test_mc_draws_zero_rows_and_shapes():
+    from pygrgl_spmv.lmm.variance import mc_draws
+
+    scales = [np.array([1.0, 0.0, 2.0]), np.array([0.0, 0.5])]
+    betas, noise = mc_draws(scales, 7, 3, trials=4, seed=5)
+    assert [b.shape for b in betas] == [(3, 4), (2, 4)] and noise.shape == (7, 4)
+    assert not betas[0][1].any() and not betas[1][0].any() and betas[0][0].all()
+    b2, n2 = mc_draws(scales, 7, 3, trials=4, seed=5)
+    np.testing.assert_array_equal(n2, noise)

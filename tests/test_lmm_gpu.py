@@ -148,3 +148,94 @@ def test_warm_start_saves_iterations(artifact_dir):
     assert t["network"]["max_rel_residual"] <= 1e-6
     assert t["network"]["max_rel_diff_vs_zero_start"] <= 1e-4
     assert t["network"]["mean_iterations"] < t["zero"]["mean_iterations"]
+
+
+# ---- variance components (MC scaling, secant search) and LOCO solves ----
+
+@pytest.fixture(scope="module")
+def dense_std(lmm_arts):
+    Gs = [L.dosages(O.load_artifact(p)) for p in lmm_arts]
+    return Gs, L.standardized(Gs)
+
+
+def _phenotype(std, seed):
+    rng = np.random.default_rng(seed)
+    Xs = [x for x, _ in std]
+    m = sum(x.shape[1] for x in Xs)
+    g = sum(x @ rng.standard_normal(x.shape[1]) for x in Xs) / np.sqrt(m)
+    return g * np.sqrt(0.4) + rng.standard_normal(Xs[0].shape[0]) * np.sqrt(0.6)
+
+
+def test_scores_and_parts_match_dense(lmm_arts, dense_std):
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+    from pygrgl_spmv.lmm import StandardizedGrgOperator
+
+    _, std = dense_std
+    with HipRuntime(build_hip_layout(lmm_arts, dtype=np.float64, requirements=_req(8))) as rt:
+        op = StandardizedGrgOperator(rt, k=8)
+        V = torch.from_numpy(np.random.default_rng(7).standard_normal((op.n, 5))).to(op.device)
+        scores = [s.cpu().numpy() for s in op.scores(V)]
+        parts = op.apply_parts(V)
+        K_V = op.apply_k(V).cpu().numpy()
+        summed = L.project(sum(p.cpu().numpy() for p in parts) / op.m_proj)
+        assert op.m_per_grg == [int(keep.sum()) for _, keep in std]
+    PV = L.project(V.cpu().numpy())
+    for (X, keep), sc, part in zip(std, scores, parts):
+        assert O.normwise_rel_err(sc[keep].T, (X.T @ PV).T) <= 1e-12
+        assert not sc[~keep].any()
+        assert O.normwise_rel_err(L.project(part.cpu().numpy()).T, (X @ (X.T @ PV)).T) <= 1e-12
+    assert O.normwise_rel_err(summed.T, K_V.T) <= 1e-13
+
+
+def test_mc_scaling_matches_dense(lmm_arts, dense_std):
+    from pygrgl_spmv.backends.hip import HipRuntime
+    from pygrgl_spmv.lmm import StandardizedGrgOperator
+    from pygrgl_spmv.lmm import variance as V
+
+    _, std = dense_std
+    Xs, masks = [x for x, _ in std], [k for _, k in std]
+    y = _phenotype(std, 11)
+    trials = 6
+    with HipRuntime(build_hip_layout(lmm_arts, dtype=np.float64, requirements=_req(8))) as rt:
+        op = StandardizedGrgOperator(rt, k=8)
+        G, E, ydev = V.mc_components(op, y, trials=trials, seed=3)
+        betas, noise = V.mc_draws(op.scales, op.n, op.m_proj, trials=trials, seed=3)
+        np.testing.assert_allclose(G.cpu().numpy(), L.genetic_values(Xs, masks, betas), atol=1e-11)
+        np.testing.assert_allclose(E.cpu().numpy(), L.project(noise), atol=1e-13)
+        for h2 in (0.25, 0.6):
+            ld = V.log_delta_from_h2(op, h2)
+            assert V.h2_from_log_delta(op, ld) == pytest.approx(h2, rel=1e-12)
+            got = V.mc_scaling(op, ydev, G, E, log_delta=ld, rel_tol=1e-12, max_iter=1000)
+            exp = L.mc_scaling(Xs, L.genetic_values(Xs, masks, betas), noise, y, ld)
+            assert got.f_reml == pytest.approx(exp["f_reml"], abs=1e-9)
+            assert got.f_jacks == pytest.approx(tuple(exp["f_jacks"]), abs=1e-9)
+            assert got.f_rands_as_data == pytest.approx(tuple(exp["f_rands_as_data"]), abs=1e-9)
+            assert got.sigma2_k == pytest.approx(exp["sigma2_k"], rel=1e-9)
+            np.testing.assert_allclose(got.hinv_y.cpu().numpy(), exp["hinv_y"], rtol=0, atol=1e-9 * np.abs(exp["hinv_y"]).max())
+
+
+def test_variance_fit_and_loco_match_dense(lmm_arts, dense_std):
+    from pygrgl_spmv.backends.hip import HipRuntime
+    from pygrgl_spmv.lmm import StandardizedGrgOperator
+    from pygrgl_spmv.lmm import variance as V
+
+    _, std = dense_std
+    Xs, masks = [x for x, _ in std], [k for _, k in std]
+    y = _phenotype(std, 12)
+    with HipRuntime(build_hip_layout(lmm_arts, dtype=np.float64, requirements=_req(8))) as rt:
+        op = StandardizedGrgOperator(rt, k=8)
+        fit = V.fit_variance_components(op, y, mc_trials=5, seed=4, rel_tol=1e-12, max_iter=1000)
+        loco = V.solve_loco(op, y, delta=fit.delta, rel_tol=1e-12, max_iter=1000).cpu().numpy()
+        betas, noise = V.mc_draws(op.scales, op.n, op.m_proj, trials=5, seed=4)
+    G = L.genetic_values(Xs, masks, betas)
+    assert 2 <= len(fit.evaluations) <= 7
+    for ev in fit.evaluations:                                   # every point of the search against the dense solve
+        assert ev.f_reml == pytest.approx(L.mc_scaling(Xs, G, noise, y, ev.log_delta)["f_reml"], abs=1e-9)
+    secant = fit.evaluations[2:] or fit.evaluations
+    assert fit.log_delta in [e.log_delta for e in fit.evaluations]
+    assert abs(next(e for e in fit.evaluations if e.log_delta == fit.log_delta).f_reml) <= min(abs(e.f_reml) for e in secant) + 1e-15
+    assert fit.h2 == pytest.approx(V.h2_from_log_delta(op, fit.log_delta)) and fit.sigma_e2 == pytest.approx(fit.delta * fit.sigma_g2)
+    exp = L.loco_solutions(Xs, y, fit.delta)
+    np.testing.assert_allclose(loco, exp, rtol=0, atol=1e-9 * np.abs(exp).max())
