@@ -4,6 +4,6 @@ The package root only re-exports CPU-safe symbols; GPU symbols live in
 ``pygrgl_spmv.backends.hip``.
 """
 
-from pygrgl_spmv.grg import BoundGRG, RuntimeRequirements
+from pygrgl_spmv.grg import BoundGRG, RuntimeRequirements, convert
 
-__all__ = ["BoundGRG", "RuntimeRequirements"]
+__all__ = ["BoundGRG", "RuntimeRequirements", "convert"]

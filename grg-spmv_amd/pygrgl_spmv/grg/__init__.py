@@ -457,4 +457,6 @@ class BoundGRG:
             return self._finish_endpoint_output(res)
 
 
-__all__ = ["BoundGRG", "Mutation", "RuntimeRequirements"]
+from pygrgl_spmv.grg.convert import convert  # noqa: E402  (convert imports this module's helpers lazily)
+
+__all__ = ["BoundGRG", "Mutation", "RuntimeRequirements", "convert"]

@@ -192,6 +192,8 @@ def compile_dag(
     coalescence_counts: np.ndarray | None = None,
     mutation_positions: np.ndarray | None = None,
     mutation_times: np.ndarray | None = None,
+    alleles=None,
+    ref_alleles=None,
 ) -> CompiledOperatorState:
     """Compile a child-list DAG (node ids topologically numbered, samples first).
 
@@ -218,8 +220,8 @@ def compile_dag(
     coal = None
     if coalescence_counts is not None:
         coal = np.asarray(coalescence_counts, dtype=np.int64)[np.asarray(node_perm, dtype=np.int64)]
-    alleles, allele_off = _encode_alleles(["A"] * M)
-    refs, ref_off = _encode_alleles(["G"] * M)
+    alleles, allele_off = _encode_alleles(["A"] * M if alleles is None else alleles)
+    refs, ref_off = _encode_alleles(["G"] * M if ref_alleles is None else ref_alleles)
     return CompiledOperatorState(
         A_blocks=blocks,
         level_offsets=level_offsets,
