@@ -134,3 +134,32 @@ def test_init_failures(case):
     for kw in bad:
         with pytest.raises((ValueError, TypeError)):
             grg.matmul(kw["x"], "up", init=kw["init"])
+
+
+@pytest.mark.parametrize("mode", ["all_streamed", "partial"])
+def test_streamed_layouts_on_fixture(fixtures, mode):
+    """The reference's GRGs with blocks streamed through the pinned ring (every block,
+    or the part a budget between the streamed and the fully resident sizes leaves)."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    path, g, X, Miss = fixtures["t200"]
+    req = full_requirements(max_k_up=4, max_k_down=4)
+    if mode == "all_streamed":
+        layout = build_hip_layout([path], requirements=req, ring_buffer_size=2, allow_residency=False)
+    else:
+        full = build_hip_layout([path], requirements=req, vram_budget_bytes=0)
+        streamed = build_hip_layout([path], requirements=req, ring_buffer_size=2, allow_residency=False)
+        budget = (streamed.bytes_total + full.required_budget_for_full_residency) // 2
+        layout = build_hip_layout([path], requirements=req, vram_budget_bytes=budget, ring_buffer_size=2)
+    flags = [b.resident for a in layout.artifacts for b in (*a.blocks_up, *a.blocks_down)]
+    assert not all(flags) and (mode == "all_streamed") == (not any(flags))
+    rng = np.random.default_rng(55)
+    with HipRuntime(layout) as rt:
+        (grg,) = rt.grgs
+        x = rng.random((4, g.num_samples))
+        np.testing.assert_allclose(grg.matmul(x, "up"), x @ X, rtol=1e-12, atol=1e-12)
+        y, m = rng.random((4, g.num_mutations)), rng.random((4, g.num_mutations))
+        np.testing.assert_allclose(grg.matmul(y, "down", miss=m), y @ X.T + m @ Miss.T, rtol=1e-12, atol=1e-12)
+        D = _diploid(X)
+        ones = np.ones((2, g.num_samples))
+        np.testing.assert_array_equal(grg.matmul(ones, "up", init="xtx"), np.tile((D * D).sum(axis=0), (2, 1)))
