@@ -419,6 +419,14 @@ __device__ __forceinline__ void gather_block(const GrgHipGatherArgs& a, int64_t 
       // block b runs on XCD b % 8; give XCD x the x-th contiguous eighth of the rows
       const int64_t q = n_light_blocks >> 3, r = n_light_blocks & 7, x = blk & 7;
       blk = x * q + (x < r ? x : r) + (blk >> 3);
+    } else if (a.flags & GRG_HIP_FLAG_XCD_RUNS) {
+      // window w of 8 * 2^s blocks: XCD x takes its x-th run of 2^s blocks, in order
+      const int s = (a.flags >> GRG_HIP_XCD_RUN_SHIFT) & 15;
+      const int64_t window = (int64_t)8 << s;
+      if (blk < (n_light_blocks / window) * window) {
+        const int64_t x = blk & 7, j = blk >> 3;
+        blk = ((j >> s) << (s + 3)) + (x << s) + (j & (((int64_t)1 << s) - 1));
+      }
     }
     const int64_t row0 = a.row_begin + (int64_t)R * (((blk * kWavesPerBlock + wave) << (6 - g_shift)) + (lane >> g_shift));
     const bool any_live = row0 < a.row_end && col < a.k;
@@ -1258,8 +1266,13 @@ int validate(const GrgHipGatherArgs& a) {
     if (!ok) return fail("(unroll, rows_per_group) must be one of (4|8|16, 1), (4|8, 2), (4, 4)");
   }
   if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_MERGED |
-                  GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_PANEL))
+                  GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_XCD_RUNS | (15 << GRG_HIP_XCD_RUN_SHIFT)))
     return fail("unknown flags");
+  if ((a.flags & (15 << GRG_HIP_XCD_RUN_SHIFT)) && !(a.flags & GRG_HIP_FLAG_XCD_RUNS))
+    return fail("a run length without GRG_HIP_FLAG_XCD_RUNS");
+  if ((a.flags & GRG_HIP_FLAG_XCD_RUNS) &&
+      (a.flags & (GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM)))
+    return fail("XCD runs apply to the per-row light path only (not with XCD_REMAP, TILED, PANEL or STREAM)");
   if (a.flags & GRG_HIP_FLAG_PANEL) {
     if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_MERGED))
       return fail("a panel launch is not tiled, merged, streamed or a partials launch");

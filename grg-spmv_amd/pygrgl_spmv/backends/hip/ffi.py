@@ -13,7 +13,7 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 11
+ABI_VERSION = 12
 F64 = 0
 F32C = 1
 FLAG_XCD_REMAP = 2
@@ -22,6 +22,8 @@ FLAG_TILED = 8
 FLAG_MERGED = 16
 FLAG_STREAM = 32
 FLAG_PANEL = 64
+FLAG_XCD_RUNS = 128
+XCD_RUN_SHIFT = 8
 TILE_CHUNK_BYTES = 128
 TILE_MAX_STAGED = 1024
 MAX_SEGS = 3

@@ -76,7 +76,17 @@ PANEL_MIN_ROWS = 1024   # ... nor levels with fewer rows
 PANEL_MAX_PASS_BYTES = 16384  # entry positions of one pass (with 32 bytes of slack) the panel kernel stages
 PANEL_MAX_PASSES = 2048       # passes of one panel (their entry offsets sit in LDS)
 DEGREE_WINDOW = 4096    # nodes per degree-sorted run of the device node order
-DEGREE_MIN_MEAN = 6.0   # levels with a lower mean up-sweep degree keep the plain locality order
+DEGREE_MIN_MEAN = 6.0
+# Run lengths (log2 of light blocks) the tuner tries for GRG_HIP_FLAG_XCD_RUNS.
+XCD_RUN_LOGS = (4, 6)
+XCD_RUN_BITS = 15 << ffi.XCD_RUN_SHIFT
+
+
+def xcd_runs_flags(log_run: int) -> int:
+    """Launch flags for XCD runs of 2^log_run light blocks (GRG_HIP_FLAG_XCD_RUNS)."""
+    if not 0 <= int(log_run) <= 15:
+        raise ValueError("log_run must be in [0, 15]")
+    return ffi.FLAG_XCD_RUNS | (int(log_run) << ffi.XCD_RUN_SHIFT)   # levels with a lower mean up-sweep degree keep the plain locality order
 
 
 def column_shapes(dtype_code: int, k: int, lds) -> list[tuple[int, int]]:
@@ -413,7 +423,7 @@ class HipBatchedSweep:
                     for i in range(n):
                         a = ffi.GrgHipGatherArgs()
                         ctypes.pointer(a)[0] = arr[i]
-                        a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_PANEL)
+                        a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_PANEL | ffi.FLAG_XCD_RUNS | XCD_RUN_BITS)
                         launches.append(a)
                         phase_of.append(p)
                         p += 1
@@ -1020,6 +1030,8 @@ class HipRuntime:
             else:
                 while a.kl > 1 and a.kl * a.cpl * self._elem() > ffi.TILE_CHUNK_BYTES:
                     a.kl //= 2
+        if a.flags & (ffi.FLAG_XCD_REMAP | ffi.FLAG_TILED | ffi.FLAG_PANEL | ffi.FLAG_STREAM):
+            a.flags &= ~(ffi.FLAG_XCD_RUNS | XCD_RUN_BITS)  # runs apply to the per-row light path alone
         if a.kl * a.es > 64 and not a.flags & ffi.FLAG_PANEL:
             a.es = 64 // a.kl
         a.out, a.ld_out, a.out_base = int(out), int(ld_out), int(out_base)
@@ -1093,6 +1105,10 @@ class HipRuntime:
                     time_shape(cpl, kl, es, unroll, rows, (fl & ffi.FLAG_MERGED) | ffi.FLAG_XCD_REMAP)
                 else:
                     time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_XCD_REMAP)
+                # XCD runs: each XCD takes runs of consecutive blocks, all XCDs in one window
+                _, cpl, kl, es, unroll, rows, fl = min(timings)
+                for log_run in XCD_RUN_LOGS:
+                    time_shape(cpl, kl, es, unroll, rows, (fl & ffi.FLAG_MERGED) | xcd_runs_flags(log_run))
                 if a.nseg == 1 or a.ekey:
                     # the streamed (software-pipelined) light path on the best column shape
                     _, cpl, kl, es, unroll, rows, fl = min(timings)

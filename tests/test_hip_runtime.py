@@ -152,6 +152,42 @@ def test_streamed_light_path(art_a, oracle_a, dtype, unroll, es):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("log_run", [0, 1, 2, 12])
+@pytest.mark.parametrize("merged", [False, True])
+def test_xcd_runs_light_path(art_a, oracle_a, dtype, log_run, merged):
+    """XCD runs of 2^log_run light blocks (windows of 8 runs; blocks past the last full
+    window, here all of them for 2^12, keep their order) on every per-row launch of
+    both sweeps, with heavy and split rows beside it: every row produced once, exact
+    for +-1 inputs, FP64 within 1e-12, FP32c within 1e-5."""
+    from pygrgl_spmv.backends.hip import HipRuntime, ffi
+    from pygrgl_spmv.backends.hip.runtime import xcd_runs_flags
+
+    layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
+    flags = xcd_runs_flags(log_run) | (ffi.FLAG_MERGED if merged else 0)
+    shape = {"unroll": 8, "rows_per_group": 1, "es": 2, "flags": flags}
+    tol = FP64_TOL if dtype == np.float64 else 1e-5
+    with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16) as rt:
+        (grg,) = rt.grgs
+        rng = np.random.default_rng(3390 + log_run)
+        for k in (1, 16):
+            for direction in ("up", "down"):
+                x = rng.choice(np.array([-1.0, 1.0]), size=(k, _cols(oracle_a, direction))).astype(dtype)
+                np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x.astype(np.float64), direction))
+        y = rng.standard_normal((5, oracle_a.num_mutations))
+        miss = rng.standard_normal((5, oracle_a.num_mutations))
+        got = grg.matmul(y.astype(dtype), "down", miss=miss.astype(dtype))
+        assert O.normwise_rel_err(got, O.matmul(oracle_a, y, "down", miss=miss)) <= tol
+
+
+def test_xcd_runs_flags_are_validated():
+    from pygrgl_spmv.backends.hip.runtime import xcd_runs_flags
+
+    assert xcd_runs_flags(4) == 128 | (4 << 8)
+    with pytest.raises(ValueError, match="log_run"):
+        xcd_runs_flags(16)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("remap", [False, True])
 @pytest.mark.parametrize("kl", [None, 1])
 def test_panel_launch(art_a, oracle_a, dtype, remap, kl):
