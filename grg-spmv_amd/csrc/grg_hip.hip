@@ -360,6 +360,27 @@ __device__ __forceinline__ void workgroup_row(const GrgHipGatherArgs& a, int64_t
   if (wave == 0 && lane < KL && live) store_row<T, CPL>(a, row, col, acc);
 }
 
+// Block order over the XCDs (block b runs on XCD b % 8). GRG_HIP_FLAG_XCD_REMAP
+// gives XCD x the x-th contiguous eighth of [0, n); GRG_HIP_FLAG_XCD_RUNS gives it
+// runs of 2^s consecutive blocks, the eight XCDs' runs side by side in one window
+// (blocks past the last full window keep their order). Both are permutations of
+// [0, n).
+__device__ __forceinline__ int64_t xcd_order(int64_t b, int64_t n, int32_t flags) {
+  if (flags & GRG_HIP_FLAG_XCD_REMAP) {
+    const int64_t q = n >> 3, r = n & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+  }
+  if (flags & GRG_HIP_FLAG_XCD_RUNS) {
+    const int s = (flags >> GRG_HIP_XCD_RUN_SHIFT) & 15;
+    const int64_t window = (int64_t)8 << s;
+    if (b < (n / window) * window) {
+      const int64_t x = b & 7, j = b >> 3;
+      return ((j >> s) << (s + 3)) + (x << s) + (j & (((int64_t)1 << s) - 1));
+    }
+  }
+  return b;
+}
+
 // Block (bx, by) of a row-gather launch: a partials item, R-row lane groups of
 // the light path, or a heavy row. Shared by the one-launch kernel and the
 // job-table kernel.
@@ -415,19 +436,7 @@ __device__ __forceinline__ void gather_block(const GrgHipGatherArgs& a, int64_t 
     const int ES = G >> kl_shift;
     const int col = (by * KL + (lg & (KL - 1))) * CPL;
     int64_t blk = bx;
-    if (a.flags & GRG_HIP_FLAG_XCD_REMAP) {
-      // block b runs on XCD b % 8; give XCD x the x-th contiguous eighth of the rows
-      const int64_t q = n_light_blocks >> 3, r = n_light_blocks & 7, x = blk & 7;
-      blk = x * q + (x < r ? x : r) + (blk >> 3);
-    } else if (a.flags & GRG_HIP_FLAG_XCD_RUNS) {
-      // window w of 8 * 2^s blocks: XCD x takes its x-th run of 2^s blocks, in order
-      const int s = (a.flags >> GRG_HIP_XCD_RUN_SHIFT) & 15;
-      const int64_t window = (int64_t)8 << s;
-      if (blk < (n_light_blocks / window) * window) {
-        const int64_t x = blk & 7, j = blk >> 3;
-        blk = ((j >> s) << (s + 3)) + (x << s) + (j & (((int64_t)1 << s) - 1));
-      }
-    }
+    blk = xcd_order(blk, n_light_blocks, a.flags);
     const int64_t row0 = a.row_begin + (int64_t)R * (((blk * kWavesPerBlock + wave) << (6 - g_shift)) + (lane >> g_shift));
     const bool any_live = row0 < a.row_end && col < a.k;
     bool live[R];
@@ -725,10 +734,7 @@ __global__ __launch_bounds__(kBlock) void tile_gather_kernel(const GrgHipGatherA
     workgroup_row<T, CPL, 8>(a, a.heavy_rows[t - nt], t - nt, (int)blockIdx.y, kl_shift, partial);
     return;
   }
-  if (a.flags & GRG_HIP_FLAG_XCD_REMAP) {
-    const int64_t q = nt >> 3, r = nt & 7, x = t & 7;
-    t = x * q + (x < r ? x : r) + (t >> 3);
-  }
+  t = xcd_order(t, nt, a.flags);
   const int32_t r0 = a.tile_row[t], r1 = a.tile_row[t + 1];
   const int32_t s0 = a.tile_src[t], s1 = a.tile_src[t + 1];
   if (s1 == s0) {
@@ -1270,9 +1276,8 @@ int validate(const GrgHipGatherArgs& a) {
     return fail("unknown flags");
   if ((a.flags & (15 << GRG_HIP_XCD_RUN_SHIFT)) && !(a.flags & GRG_HIP_FLAG_XCD_RUNS))
     return fail("a run length without GRG_HIP_FLAG_XCD_RUNS");
-  if ((a.flags & GRG_HIP_FLAG_XCD_RUNS) &&
-      (a.flags & (GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM)))
-    return fail("XCD runs apply to the per-row light path only (not with XCD_REMAP, TILED, PANEL or STREAM)");
+  if ((a.flags & GRG_HIP_FLAG_XCD_RUNS) && (a.flags & (GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM)))
+    return fail("XCD runs apply to the per-row light path and to tiles (not with XCD_REMAP, PANEL or STREAM)");
   if (a.flags & GRG_HIP_FLAG_PANEL) {
     if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_MERGED))
       return fail("a panel launch is not tiled, merged, streamed or a partials launch");

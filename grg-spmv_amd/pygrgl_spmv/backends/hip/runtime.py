@@ -78,7 +78,7 @@ PANEL_MAX_PASSES = 2048       # passes of one panel (their entry offsets sit in 
 DEGREE_WINDOW = 4096    # nodes per degree-sorted run of the device node order
 DEGREE_MIN_MEAN = 6.0
 # Run lengths (log2 of light blocks) the tuner tries for GRG_HIP_FLAG_XCD_RUNS.
-XCD_RUN_LOGS = (4, 6)
+XCD_RUN_LOGS = (4, 6, 9, 12)
 XCD_RUN_BITS = 15 << ffi.XCD_RUN_SHIFT
 
 
@@ -1030,8 +1030,8 @@ class HipRuntime:
             else:
                 while a.kl > 1 and a.kl * a.cpl * self._elem() > ffi.TILE_CHUNK_BYTES:
                     a.kl //= 2
-        if a.flags & (ffi.FLAG_XCD_REMAP | ffi.FLAG_TILED | ffi.FLAG_PANEL | ffi.FLAG_STREAM):
-            a.flags &= ~(ffi.FLAG_XCD_RUNS | XCD_RUN_BITS)  # runs apply to the per-row light path alone
+        if a.flags & (ffi.FLAG_XCD_REMAP | ffi.FLAG_PANEL | ffi.FLAG_STREAM):
+            a.flags &= ~(ffi.FLAG_XCD_RUNS | XCD_RUN_BITS)  # runs apply to the light path and to tiles
         if a.kl * a.es > 64 and not a.flags & ffi.FLAG_PANEL:
             a.es = 64 // a.kl
         a.out, a.ld_out, a.out_base = int(out), int(ld_out), int(out_base)
@@ -1143,6 +1143,8 @@ class HipRuntime:
                     if len(timings) > n_untiled:
                         _, cpl, kl, es, unroll, rows, _ = min(timings[n_untiled:])
                         time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_TILED | ffi.FLAG_XCD_REMAP)
+                        for log_run in XCD_RUN_LOGS:
+                            time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_TILED | xcd_runs_flags(log_run))
             top = min(timings)
             best = top[1:]
             self._tune_cache[key] = best

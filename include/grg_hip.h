@@ -69,13 +69,13 @@ enum GrgHipDtype {
 #define GRG_HIP_FLAG_STREAM 32
 /* Panel launch: see the panel fields of GrgHipGatherArgs. */
 #define GRG_HIP_FLAG_PANEL 64
-/* Light path of the per-row launch only: XCD x (blocks b with b % 8 == x) takes
- * runs of 2^((flags >> GRG_HIP_XCD_RUN_SHIFT) & 15) consecutive light blocks,
- * the eight XCDs' runs side by side, so every XCD's L2 sees consecutive rows
- * while all XCDs move through one window of rows together (their shared
- * sources stay in the memory-side cache). Blocks past the last full window
- * keep their order. Not combined with GRG_HIP_FLAG_XCD_REMAP, TILED, PANEL or
- * STREAM. No reference counterpart. */
+/* Light path of the per-row launch, and the tiles of a tiled launch: XCD x
+ * (blocks b with b % 8 == x) takes runs of 2^((flags >> GRG_HIP_XCD_RUN_SHIFT)
+ * & 15) consecutive blocks, the eight XCDs' runs side by side, so every XCD's
+ * L2 sees consecutive rows while all XCDs move through one window of rows
+ * together (their shared sources stay in the memory-side cache). Blocks past
+ * the last full window keep their order. Not combined with
+ * GRG_HIP_FLAG_XCD_REMAP, PANEL or STREAM. No reference counterpart. */
 #define GRG_HIP_FLAG_XCD_RUNS 128
 #define GRG_HIP_XCD_RUN_SHIFT 8
 /* Bytes of one staged row (kl * cpl columns) in a tiled launch, at most. */

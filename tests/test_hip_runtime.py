@@ -153,17 +153,18 @@ def test_streamed_light_path(art_a, oracle_a, dtype, unroll, es):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("log_run", [0, 1, 2, 12])
-@pytest.mark.parametrize("merged", [False, True])
-def test_xcd_runs_light_path(art_a, oracle_a, dtype, log_run, merged):
-    """XCD runs of 2^log_run light blocks (windows of 8 runs; blocks past the last full
-    window, here all of them for 2^12, keep their order) on every per-row launch of
-    both sweeps, with heavy and split rows beside it: every row produced once, exact
-    for +-1 inputs, FP64 within 1e-12, FP32c within 1e-5."""
+@pytest.mark.parametrize("form", ["light", "merged", "tiled"])
+def test_xcd_runs_light_path(art_a, oracle_a, dtype, log_run, form):
+    """XCD runs of 2^log_run blocks (windows of 8 runs; blocks past the last full
+    window, here all of them for 2^12, keep their order) on every per-row light path
+    (segment by segment or merged) or tiled launch of both sweeps, with heavy and
+    split rows beside it: every row produced once, exact for +-1 inputs, FP64 within
+    1e-12, FP32c within 1e-5."""
     from pygrgl_spmv.backends.hip import HipRuntime, ffi
     from pygrgl_spmv.backends.hip.runtime import xcd_runs_flags
 
     layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
-    flags = xcd_runs_flags(log_run) | (ffi.FLAG_MERGED if merged else 0)
+    flags = xcd_runs_flags(log_run) | {"light": 0, "merged": ffi.FLAG_MERGED, "tiled": ffi.FLAG_TILED}[form]
     shape = {"unroll": 8, "rows_per_group": 1, "es": 2, "flags": flags}
     tol = FP64_TOL if dtype == np.float64 else 1e-5
     with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16) as rt:
