@@ -13,6 +13,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+from oracle import grg_oracle as O
 from pygrgl_spmv import convert
 from pygrgl_spmv.grg.grgl_file import read_grg
 from tests._builders import build_hip_layout, full_requirements
@@ -163,3 +164,26 @@ def test_streamed_layouts_on_fixture(fixtures, mode):
         D = _diploid(X)
         ones = np.ones((2, g.num_samples))
         np.testing.assert_array_equal(grg.matmul(ones, "up", init="xtx"), np.tile((D * D).sum(axis=0), (2, 1)))
+
+
+@pytest.mark.parametrize("name", ["t200", "msprime"])
+def test_fp32c_on_fixture(fixtures, name):
+    """The FP32-compensated form on the reference's GRGs: exact for +-1 inputs (integer
+    sums below 2^24), within 1e-5 normwise for random inputs, missing data included."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    path, g, X, Miss = fixtures[name]
+    rng = np.random.default_rng(77)
+    layout = build_hip_layout([path], dtype=np.float32, requirements=full_requirements(max_k_up=8, max_k_down=8))
+    with HipRuntime(layout) as rt:
+        (grg,) = rt.grgs
+        s = rng.choice(np.array([-1.0, 1.0]), size=(8, g.num_samples))
+        np.testing.assert_array_equal(grg.matmul(s.astype(np.float32), "up"), (s @ X).astype(np.float32))
+        t = rng.choice(np.array([-1.0, 1.0]), size=(8, g.num_mutations))
+        np.testing.assert_array_equal(grg.matmul(t.astype(np.float32), "down"), (t @ X.T).astype(np.float32))
+        y, m = rng.standard_normal((4, g.num_mutations)), rng.standard_normal((4, g.num_mutations))
+        got = grg.matmul(y.astype(np.float32), "down", miss=m.astype(np.float32))
+        assert O.normwise_rel_err(got, y @ X.T + m @ Miss.T) <= 1e-5
+        x = rng.standard_normal((4, g.num_individuals))
+        got = grg.matmul(x.astype(np.float32), "up", by_individual=True)
+        assert O.normwise_rel_err(got, x @ _diploid(X)) <= 1e-5
