@@ -1,6 +1,6 @@
 """Several GRGs (one per chromosome) in one HIP runtime, captured into one hipGraph.
 
-    python -m pygrgl_spmv.chromosomes [--chromosomes 22 --samples 200000 --budget-gb 32 --ring 4]
+    PYTHONPATH=grg-spmv_amd python -m pygrgl_spmv.chromosomes [--chromosomes 22 --samples 200000 --budget-gb 32 --ring 4]
 
 The batched configuration of the benchmark suite: per-chromosome synthetic GRGs
 over the same samples with uneven mutation counts (U[20k, 400k]), one runtime
