@@ -76,7 +76,7 @@ def score_square_sums(op, Z):
 
     c = int(Z.shape[1])
     total = torch.zeros((c,), dtype=torch.float64, device=op.device)
-    for g, sc in enumerate(op.scores(Z)):
+    for sc in op.scores(Z):
         total += (sc * sc).sum(dim=0)
     return total
 

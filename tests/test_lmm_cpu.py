@@ -226,5 +226,5 @@ def test_mc_draws_zero_rows_and_shapes():
     betas, noise = mc_draws(scales, 7, 3, trials=4, seed=5)
     assert [b.shape for b in betas] == [(3, 4), (2, 4)] and noise.shape == (7, 4)
     assert not betas[0][1].any() and not betas[1][0].any() and betas[0][0].all()
-    b2, n2 = mc_draws(scales, 7, 3, trials=4, seed=5)
+    _, n2 = mc_draws(scales, 7, 3, trials=4, seed=5)
     np.testing.assert_array_equal(n2, noise)
