@@ -251,6 +251,59 @@ __device__ __forceinline__ void gather_rows_keyed(const uint32_t* __restrict__ e
   }
 }
 
+// gather_rows_keyed for one row per wave (kl * es = 64 lanes, R = 1): the wave
+// loads each round's U * es consecutive keys together, one dword per lane per 64
+// keys, and hands lane (e, c) its keys with lane shuffles, in place of U key
+// loads per lane. Lane e still sums entries e, e + es, e + 2 es, ... in the
+// same rounds, so the sums are bit for bit those of gather_rows_keyed. The loop
+// bounds are the wave's, so every lane takes part in every shuffle; lanes past
+// the last column chunk (col_ok false) load keys but gather nothing.
+template <typename T, int CPL, int U, typename Acc>
+__device__ __forceinline__ void gather_row_keyed_wave(const uint32_t* __restrict__ ekey, const T* src0, const T* src1,
+                                                      const T* src2, int64_t ld0, int64_t ld1, int64_t ld2, int32_t begin,
+                                                      int32_t end, int e, int es, int kl_shift, int lane, bool col_ok,
+                                                      Acc (&acc)[CPL]) {
+  const int kl_mask = (1 << kl_shift) - 1;
+  const int nslot = (U >> kl_shift) > 0 ? (U >> kl_shift) : 1;  // 64-key loads per round
+  for (int32_t b = begin; b < end; b += U * es) {
+    uint32_t ks[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int32_t idx = b + q * 64 + lane;
+      ks[q] = (q < nslot && idx < end) ? ekey[idx] : 0xffffffffu;
+    }
+    uint32_t kk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      // entry b + e + u * es was loaded by lane e + (u mod kl) * es into slot u / kl
+      const int slot = u >> kl_shift;
+      uint32_t v = ks[0];
+#pragma unroll
+      for (int q = 1; q < U; ++q)
+        if (q == slot) v = ks[q];
+      kk[u] = __shfl(v, e + (u & kl_mask) * es, 64);
+    }
+    T v[U][CPL];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t sg = kk[u] >> 30;
+      if (col_ok && sg != 3u) {
+        const int64_t r = (int64_t)(kk[u] & 0x3fffffffu);
+        const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
+        const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
+        load_raw<T, CPL>(base + r * ld, v[u]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) v[u][c] = T(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) acc[c].add(v[u][c]);
+  }
+}
+
 // Row-wise seed / epilogue terms (added once per row by one lane).
 template <typename T, int CPL, typename Acc>
 __device__ __forceinline__ void add_row_terms(const GrgHipGatherArgs& a, int64_t row, int col, Acc (&acc)[CPL]) {
@@ -447,6 +500,27 @@ __device__ __forceinline__ void gather_block(const GrgHipGatherArgs& a, int64_t 
 #pragma unroll
       for (int c = 0; c < CPL; ++c) racc[t][c].zero();
     }
+    if constexpr (R == 1) {
+      if ((a.flags & GRG_HIP_FLAG_WAVE_KEYS) && g_shift == 6) {
+        // one row per wave (row0 is the wave's): keys loaded by the whole wave
+        if (row0 < a.row_end) {
+          const int32_t p0 = a.eptr[row0], p1 = a.eptr[row0 + 1];
+          const bool light = !(a.heavy_threshold >= 0 && p1 - p0 > a.heavy_threshold);
+          const bool col_ok = col < a.k;
+          live[0] = light && col_ok;
+          const T* src0 = static_cast<const T*>(a.seg[0].src) + col;
+          const T* src1 = (a.nseg > 1 ? static_cast<const T*>(a.seg[1].src) : static_cast<const T*>(a.seg[0].src)) + col;
+          const T* src2 = (a.nseg > 2 ? static_cast<const T*>(a.seg[2].src) : static_cast<const T*>(a.seg[0].src)) + col;
+          const int64_t ld0 = a.seg[0].ld;
+          const int64_t ld1 = a.nseg > 1 ? a.seg[1].ld : ld0;
+          const int64_t ld2 = a.nseg > 2 ? a.seg[2].ld : ld0;
+          gather_row_keyed_wave<T, CPL, U>(a.ekey, src0, src1, src2, ld0, ld1, ld2, p0, light ? p1 : p0, e, ES, kl_shift,
+                                           lane, col_ok, racc[0]);
+          if (e == 0 && live[0]) add_row_terms<T, CPL>(a, row0, col, racc[0]);
+        }
+        goto reduce_and_store;
+      }
+    }
     if (any_live && (a.flags & GRG_HIP_FLAG_MERGED)) {
       // one pointer pair per row over the merged entries of all segments
       int32_t pb[R + 1];
@@ -508,6 +582,7 @@ __device__ __forceinline__ void gather_block(const GrgHipGatherArgs& a, int64_t 
           if (live[t]) add_row_terms<T, CPL>(a, row0 + t, col, racc[t]);
       }
     }
+  reduce_and_store:
     for (int off = KL; off < G; off <<= 1) {
 #pragma unroll
       for (int t = 0; t < R; ++t)
@@ -1272,8 +1347,15 @@ int validate(const GrgHipGatherArgs& a) {
     if (!ok) return fail("(unroll, rows_per_group) must be one of (4|8|16, 1), (4|8, 2), (4, 4)");
   }
   if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_MERGED |
-                  GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_XCD_RUNS | (15 << GRG_HIP_XCD_RUN_SHIFT)))
+                  GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_XCD_RUNS | (15 << GRG_HIP_XCD_RUN_SHIFT) |
+                  GRG_HIP_FLAG_WAVE_KEYS))
     return fail("unknown flags");
+  if (a.flags & GRG_HIP_FLAG_WAVE_KEYS) {
+    if (!(a.flags & GRG_HIP_FLAG_MERGED) || (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM)))
+      return fail("wave-loaded keys need the merged light path (not tiled, panel or streamed)");
+    if (a.kl * a.es != 64 || (a.rows_per_group != 0 && a.rows_per_group != 1))
+      return fail("wave-loaded keys need kl * es = 64 and one row per lane group");
+  }
   if ((a.flags & (15 << GRG_HIP_XCD_RUN_SHIFT)) && !(a.flags & GRG_HIP_FLAG_XCD_RUNS))
     return fail("a run length without GRG_HIP_FLAG_XCD_RUNS");
   if ((a.flags & GRG_HIP_FLAG_XCD_RUNS) && (a.flags & (GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM)))

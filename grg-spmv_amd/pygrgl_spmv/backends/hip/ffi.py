@@ -13,7 +13,7 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 12
+ABI_VERSION = 13
 F64 = 0
 F32C = 1
 FLAG_XCD_REMAP = 2
@@ -24,6 +24,7 @@ FLAG_STREAM = 32
 FLAG_PANEL = 64
 FLAG_XCD_RUNS = 128
 XCD_RUN_SHIFT = 8
+FLAG_WAVE_KEYS = 4096
 TILE_CHUNK_BYTES = 128
 TILE_MAX_STAGED = 1024
 MAX_SEGS = 3

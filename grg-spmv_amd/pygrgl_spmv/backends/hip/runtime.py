@@ -423,7 +423,8 @@ class HipBatchedSweep:
                     for i in range(n):
                         a = ffi.GrgHipGatherArgs()
                         ctypes.pointer(a)[0] = arr[i]
-                        a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_PANEL | ffi.FLAG_XCD_RUNS | XCD_RUN_BITS)
+                        a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_PANEL | ffi.FLAG_XCD_RUNS | XCD_RUN_BITS |
+                                     ffi.FLAG_WAVE_KEYS)
                         launches.append(a)
                         phase_of.append(p)
                         p += 1
@@ -1034,6 +1035,9 @@ class HipRuntime:
             a.flags &= ~(ffi.FLAG_XCD_RUNS | XCD_RUN_BITS)  # runs apply to the light path and to tiles
         if a.kl * a.es > 64 and not a.flags & ffi.FLAG_PANEL:
             a.es = 64 // a.kl
+        if (a.flags & ffi.FLAG_WAVE_KEYS) and (not a.flags & ffi.FLAG_MERGED or a.kl * a.es != 64 or a.rows_per_group > 1
+                                               or a.flags & (ffi.FLAG_TILED | ffi.FLAG_PANEL | ffi.FLAG_STREAM)):
+            a.flags &= ~ffi.FLAG_WAVE_KEYS  # wave-loaded keys need one merged row per wave
         a.out, a.ld_out, a.out_base = int(out), int(ld_out), int(out_base)
         a.nseg = len(segs)
         for i, (ptr, idx, src, ld, base) in enumerate(segs):
@@ -1102,13 +1106,18 @@ class HipRuntime:
                     for es_m in sorted({es, max(1, es // 2), min(64 // kl, es * 2)}):
                         time_shape(cpl, kl, es_m, unroll, rows, ffi.FLAG_MERGED)
                     _, cpl, kl, es, unroll, rows, fl = min(timings)
-                    time_shape(cpl, kl, es, unroll, rows, (fl & ffi.FLAG_MERGED) | ffi.FLAG_XCD_REMAP)
+                    if fl & ffi.FLAG_MERGED:
+                        # one merged row per wave, its keys loaded by the whole wave
+                        for u in sorted({unroll, 8}):
+                            time_shape(cpl, kl, 64 // kl, u, 1, ffi.FLAG_MERGED | ffi.FLAG_WAVE_KEYS)
+                        _, cpl, kl, es, unroll, rows, fl = min(timings)
+                    time_shape(cpl, kl, es, unroll, rows, (fl & (ffi.FLAG_MERGED | ffi.FLAG_WAVE_KEYS)) | ffi.FLAG_XCD_REMAP)
                 else:
                     time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_XCD_REMAP)
                 # XCD runs: each XCD takes runs of consecutive blocks, all XCDs in one window
                 _, cpl, kl, es, unroll, rows, fl = min(timings)
                 for log_run in XCD_RUN_LOGS:
-                    time_shape(cpl, kl, es, unroll, rows, (fl & ffi.FLAG_MERGED) | xcd_runs_flags(log_run))
+                    time_shape(cpl, kl, es, unroll, rows, (fl & (ffi.FLAG_MERGED | ffi.FLAG_WAVE_KEYS)) | xcd_runs_flags(log_run))
                 if a.nseg == 1 or a.ekey:
                     # the streamed (software-pipelined) light path on the best column shape
                     _, cpl, kl, es, unroll, rows, fl = min(timings)
