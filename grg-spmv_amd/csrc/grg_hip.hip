@@ -251,38 +251,48 @@ __device__ __forceinline__ void gather_rows_keyed(const uint32_t* __restrict__ e
   }
 }
 
-// gather_rows_keyed for one row per wave (kl * es = 64 lanes, R = 1): the wave
-// loads each round's U * es consecutive keys together, one dword per lane per 64
-// keys, and hands lane (e, c) its keys with lane shuffles, in place of U key
-// loads per lane. Lane e still sums entries e, e + es, e + 2 es, ... in the
-// same rounds, so the sums are bit for bit those of gather_rows_keyed. The loop
-// bounds are the wave's, so every lane takes part in every shuffle; lanes past
-// the last column chunk (col_ok false) load keys but gather nothing.
-template <typename T, int CPL, int U, typename Acc>
-__device__ __forceinline__ void gather_row_keyed_wave(const uint32_t* __restrict__ ekey, const T* src0, const T* src1,
-                                                      const T* src2, int64_t ld0, int64_t ld1, int64_t ld2, int32_t begin,
-                                                      int32_t end, int e, int es, int kl_shift, int lane, bool col_ok,
-                                                      Acc (&acc)[CPL]) {
+// One row per lane group (G lanes, R = 1): the group loads each round's U * es
+// consecutive entry keys (merged entries) or source indices (one segment)
+// together, one dword per lane per G entries, and hands lane (e, c) its entries
+// with lane shuffles, in place of U loads per lane. Lane e still sums entries
+// e, e + es, e + 2 es, ... in the same rounds, so the sums are bit for bit those
+// of gather_rows_keyed / gather_rows. The loop bounds are the group's, so every
+// shuffle reads an active lane of the same group; lanes past the last column
+// chunk (col_ok false) load entries but gather nothing.
+template <int U>
+__device__ __forceinline__ void group_entries(const uint32_t* __restrict__ ent, int32_t b, int32_t end, int e, int es,
+                                              int kl_shift, int lane, int g_shift, uint32_t (&kk)[U]) {
+  const int G = 1 << g_shift;
+  const int lg = lane & (G - 1);
+  const int group0 = lane - lg;
   const int kl_mask = (1 << kl_shift) - 1;
-  const int nslot = (U >> kl_shift) > 0 ? (U >> kl_shift) : 1;  // 64-key loads per round
+  const int nslot = (U >> kl_shift) > 0 ? (U >> kl_shift) : 1;  // G-entry loads per round
+  uint32_t ks[U];
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int32_t i = b + q * G + lg;
+    ks[q] = (q < nslot && i < end) ? ent[i] : 0xffffffffu;
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    // entry b + e + u * es was loaded by group lane e + (u mod kl) * es into slot u / kl
+    const int slot = u >> kl_shift;
+    uint32_t v = ks[0];
+#pragma unroll
+    for (int q = 1; q < U; ++q)
+      if (q == slot) v = ks[q];
+    kk[u] = __shfl(v, group0 + e + (u & kl_mask) * es, 64);
+  }
+}
+
+template <typename T, int CPL, int U, typename Acc>
+__device__ __forceinline__ void gather_row_keyed_group(const uint32_t* __restrict__ ekey, const T* src0, const T* src1,
+                                                       const T* src2, int64_t ld0, int64_t ld1, int64_t ld2, int32_t begin,
+                                                       int32_t end, int e, int es, int kl_shift, int lane, int g_shift,
+                                                       bool col_ok, Acc (&acc)[CPL]) {
   for (int32_t b = begin; b < end; b += U * es) {
-    uint32_t ks[U];
-#pragma unroll
-    for (int q = 0; q < U; ++q) {
-      const int32_t idx = b + q * 64 + lane;
-      ks[q] = (q < nslot && idx < end) ? ekey[idx] : 0xffffffffu;
-    }
     uint32_t kk[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      // entry b + e + u * es was loaded by lane e + (u mod kl) * es into slot u / kl
-      const int slot = u >> kl_shift;
-      uint32_t v = ks[0];
-#pragma unroll
-      for (int q = 1; q < U; ++q)
-        if (q == slot) v = ks[q];
-      kk[u] = __shfl(v, e + (u & kl_mask) * es, 64);
-    }
+    group_entries<U>(ekey, b, end, e, es, kl_shift, lane, g_shift, kk);
     T v[U][CPL];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -292,6 +302,30 @@ __device__ __forceinline__ void gather_row_keyed_wave(const uint32_t* __restrict
         const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
         const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
         load_raw<T, CPL>(base + r * ld, v[u]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) v[u][c] = T(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) acc[c].add(v[u][c]);
+  }
+}
+
+template <typename T, int CPL, int U, typename Acc>
+__device__ __forceinline__ void gather_row_indexed_group(const int32_t* __restrict__ idx, const T* src, int64_t ld,
+                                                         int32_t begin, int32_t end, int e, int es, int kl_shift, int lane,
+                                                         int g_shift, bool col_ok, Acc (&acc)[CPL]) {
+  for (int32_t b = begin; b < end; b += U * es) {
+    uint32_t kk[U];
+    group_entries<U>(reinterpret_cast<const uint32_t*>(idx), b, end, e, es, kl_shift, lane, g_shift, kk);
+    T v[U][CPL];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (col_ok && kk[u] != 0xffffffffu) {
+        load_raw<T, CPL>(src + (int64_t)kk[u] * ld, v[u]);
       } else {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) v[u][c] = T(0);
@@ -501,21 +535,28 @@ __device__ __forceinline__ void gather_block(const GrgHipGatherArgs& a, int64_t 
       for (int c = 0; c < CPL; ++c) racc[t][c].zero();
     }
     if constexpr (R == 1) {
-      if ((a.flags & GRG_HIP_FLAG_WAVE_KEYS) && g_shift == 6) {
-        // one row per wave (row0 is the wave's): keys loaded by the whole wave
+      if (a.flags & GRG_HIP_FLAG_GROUP_KEYS) {
+        // one row per lane group (row0 is the group's): entries loaded by the whole group
         if (row0 < a.row_end) {
-          const int32_t p0 = a.eptr[row0], p1 = a.eptr[row0 + 1];
+          const bool merged = (a.flags & GRG_HIP_FLAG_MERGED) != 0;
+          const int32_t* ptr = merged ? a.eptr : a.seg[0].ptr - a.seg[0].ptr_base;
+          const int32_t p0 = ptr[row0], p1 = ptr[row0 + 1];
           const bool light = !(a.heavy_threshold >= 0 && p1 - p0 > a.heavy_threshold);
           const bool col_ok = col < a.k;
           live[0] = light && col_ok;
           const T* src0 = static_cast<const T*>(a.seg[0].src) + col;
-          const T* src1 = (a.nseg > 1 ? static_cast<const T*>(a.seg[1].src) : static_cast<const T*>(a.seg[0].src)) + col;
-          const T* src2 = (a.nseg > 2 ? static_cast<const T*>(a.seg[2].src) : static_cast<const T*>(a.seg[0].src)) + col;
-          const int64_t ld0 = a.seg[0].ld;
-          const int64_t ld1 = a.nseg > 1 ? a.seg[1].ld : ld0;
-          const int64_t ld2 = a.nseg > 2 ? a.seg[2].ld : ld0;
-          gather_row_keyed_wave<T, CPL, U>(a.ekey, src0, src1, src2, ld0, ld1, ld2, p0, light ? p1 : p0, e, ES, kl_shift,
-                                           lane, col_ok, racc[0]);
+          if (merged) {
+            const T* src1 = (a.nseg > 1 ? static_cast<const T*>(a.seg[1].src) : static_cast<const T*>(a.seg[0].src)) + col;
+            const T* src2 = (a.nseg > 2 ? static_cast<const T*>(a.seg[2].src) : static_cast<const T*>(a.seg[0].src)) + col;
+            const int64_t ld0 = a.seg[0].ld;
+            const int64_t ld1 = a.nseg > 1 ? a.seg[1].ld : ld0;
+            const int64_t ld2 = a.nseg > 2 ? a.seg[2].ld : ld0;
+            gather_row_keyed_group<T, CPL, U>(a.ekey, src0, src1, src2, ld0, ld1, ld2, p0, light ? p1 : p0, e, ES, kl_shift,
+                                              lane, g_shift, col_ok, racc[0]);
+          } else {
+            gather_row_indexed_group<T, CPL, U>(a.seg[0].idx, src0, a.seg[0].ld, p0, light ? p1 : p0, e, ES, kl_shift, lane,
+                                                g_shift, col_ok, racc[0]);
+          }
           if (e == 0 && live[0]) add_row_terms<T, CPL>(a, row0, col, racc[0]);
         }
         goto reduce_and_store;
@@ -1348,13 +1389,14 @@ int validate(const GrgHipGatherArgs& a) {
   }
   if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_MERGED |
                   GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_XCD_RUNS | (15 << GRG_HIP_XCD_RUN_SHIFT) |
-                  GRG_HIP_FLAG_WAVE_KEYS))
+                  GRG_HIP_FLAG_GROUP_KEYS))
     return fail("unknown flags");
-  if (a.flags & GRG_HIP_FLAG_WAVE_KEYS) {
-    if (!(a.flags & GRG_HIP_FLAG_MERGED) || (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM)))
-      return fail("wave-loaded keys need the merged light path (not tiled, panel or streamed)");
-    if (a.kl * a.es != 64 || (a.rows_per_group != 0 && a.rows_per_group != 1))
-      return fail("wave-loaded keys need kl * es = 64 and one row per lane group");
+  if (a.flags & GRG_HIP_FLAG_GROUP_KEYS) {
+    if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_PARTIALS))
+      return fail("group-loaded entries apply to the per-row light path (not tiled, panel, streamed or partials)");
+    if (!(a.flags & GRG_HIP_FLAG_MERGED) && a.nseg != 1)
+      return fail("group-loaded entries need merged entries or one segment");
+    if (a.rows_per_group != 0 && a.rows_per_group != 1) return fail("group-loaded entries need one row per lane group");
   }
   if ((a.flags & (15 << GRG_HIP_XCD_RUN_SHIFT)) && !(a.flags & GRG_HIP_FLAG_XCD_RUNS))
     return fail("a run length without GRG_HIP_FLAG_XCD_RUNS");

@@ -24,7 +24,7 @@ FLAG_STREAM = 32
 FLAG_PANEL = 64
 FLAG_XCD_RUNS = 128
 XCD_RUN_SHIFT = 8
-FLAG_WAVE_KEYS = 4096
+FLAG_GROUP_KEYS = 4096
 TILE_CHUNK_BYTES = 128
 TILE_MAX_STAGED = 1024
 MAX_SEGS = 3

@@ -424,7 +424,7 @@ class HipBatchedSweep:
                         a = ffi.GrgHipGatherArgs()
                         ctypes.pointer(a)[0] = arr[i]
                         a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_PANEL | ffi.FLAG_XCD_RUNS | XCD_RUN_BITS |
-                                     ffi.FLAG_WAVE_KEYS)
+                                     ffi.FLAG_GROUP_KEYS)
                         launches.append(a)
                         phase_of.append(p)
                         p += 1
@@ -1035,9 +1035,9 @@ class HipRuntime:
             a.flags &= ~(ffi.FLAG_XCD_RUNS | XCD_RUN_BITS)  # runs apply to the light path and to tiles
         if a.kl * a.es > 64 and not a.flags & ffi.FLAG_PANEL:
             a.es = 64 // a.kl
-        if (a.flags & ffi.FLAG_WAVE_KEYS) and (not a.flags & ffi.FLAG_MERGED or a.kl * a.es != 64 or a.rows_per_group > 1
-                                               or a.flags & (ffi.FLAG_TILED | ffi.FLAG_PANEL | ffi.FLAG_STREAM)):
-            a.flags &= ~ffi.FLAG_WAVE_KEYS  # wave-loaded keys need one merged row per wave
+        if (a.flags & ffi.FLAG_GROUP_KEYS) and ((not a.flags & ffi.FLAG_MERGED and a.nseg != 1) or a.rows_per_group > 1
+                                                or a.flags & (ffi.FLAG_TILED | ffi.FLAG_PANEL | ffi.FLAG_STREAM)):
+            a.flags &= ~ffi.FLAG_GROUP_KEYS  # group-loaded entries need one row per lane group, merged or one segment
         a.out, a.ld_out, a.out_base = int(out), int(ld_out), int(out_base)
         a.nseg = len(segs)
         for i, (ptr, idx, src, ld, base) in enumerate(segs):
@@ -1070,9 +1070,11 @@ class HipRuntime:
         (write-out launches, whose rows are the caller-visible arena), a temporary
         buffer of the same shape). Stage 1 times every
         admissible (cpl, kl, es) shape with 8 gathers in flight and one row per lane
-        group; stage 2 tries the other (gathers in flight, rows per group) pairs and
-        the XCD remap on the best shape. The fastest launch is cached per artifact,
-        sweep configuration, level and k."""
+        group; the later stages try, each on the best launch so far, the other
+        (gathers in flight, rows per group) pairs, merged entries, entries loaded by
+        the whole lane group, the XCD remap and XCD runs, the streamed form, panels
+        and the tiled form. The fastest launch is cached per artifact, sweep
+        configuration, level and k."""
         torch = _torch()
         best = self._tune_cache.get(key)
         if best is None and self.autotune and a.row_end > a.row_begin:
@@ -1106,18 +1108,24 @@ class HipRuntime:
                     for es_m in sorted({es, max(1, es // 2), min(64 // kl, es * 2)}):
                         time_shape(cpl, kl, es_m, unroll, rows, ffi.FLAG_MERGED)
                     _, cpl, kl, es, unroll, rows, fl = min(timings)
-                    if fl & ffi.FLAG_MERGED:
-                        # one merged row per wave, its keys loaded by the whole wave
-                        for u in sorted({unroll, 8}):
-                            time_shape(cpl, kl, 64 // kl, u, 1, ffi.FLAG_MERGED | ffi.FLAG_WAVE_KEYS)
-                        _, cpl, kl, es, unroll, rows, fl = min(timings)
-                    time_shape(cpl, kl, es, unroll, rows, (fl & (ffi.FLAG_MERGED | ffi.FLAG_WAVE_KEYS)) | ffi.FLAG_XCD_REMAP)
-                else:
-                    time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_XCD_REMAP)
-                # XCD runs: each XCD takes runs of consecutive blocks, all XCDs in one window
-                _, cpl, kl, es, unroll, rows, fl = min(timings)
-                for log_run in XCD_RUN_LOGS:
-                    time_shape(cpl, kl, es, unroll, rows, (fl & (ffi.FLAG_MERGED | ffi.FLAG_WAVE_KEYS)) | xcd_runs_flags(log_run))
+                    # one merged row per lane group, its keys loaded by the whole group
+                    for es_g in sorted({es, 64 // kl}):
+                        time_shape(cpl, kl, es_g, unroll, 1, ffi.FLAG_MERGED | ffi.FLAG_GROUP_KEYS)
+                elif a.nseg == 1:
+                    # one row per lane group, its indices loaded by the whole group
+                    time_shape(cpl, kl, es, unroll, 1, ffi.FLAG_GROUP_KEYS)
+                # block orders over the XCDs (the remap, and runs of consecutive blocks with all
+                # XCDs in one window), on the best launch so far and on the best with
+                # group-loaded entries: the orders change which of the two wins
+                starts = [min(timings)]
+                grouped = [t for t in timings if t[6] & ffi.FLAG_GROUP_KEYS]
+                if grouped and min(grouped)[1:] != starts[0][1:]:
+                    starts.append(min(grouped))
+                for _, cpl, kl, es, unroll, rows, fl in starts:
+                    keep = fl & (ffi.FLAG_MERGED | ffi.FLAG_GROUP_KEYS)
+                    time_shape(cpl, kl, es, unroll, rows, keep | ffi.FLAG_XCD_REMAP)
+                    for log_run in XCD_RUN_LOGS:
+                        time_shape(cpl, kl, es, unroll, rows, keep | xcd_runs_flags(log_run))
                 if a.nseg == 1 or a.ekey:
                     # the streamed (software-pipelined) light path on the best column shape
                     _, cpl, kl, es, unroll, rows, fl = min(timings)

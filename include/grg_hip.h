@@ -78,11 +78,13 @@ enum GrgHipDtype {
  * GRG_HIP_FLAG_XCD_REMAP, PANEL or STREAM. No reference counterpart. */
 #define GRG_HIP_FLAG_XCD_RUNS 128
 #define GRG_HIP_XCD_RUN_SHIFT 8
-/* Merged light path with one row per wave (kl * es = 64, rows_per_group 1): the
- * wave loads each round's entry keys together (one dword per lane per 64 keys)
- * and hands them out with lane shuffles, instead of one key load per lane and
- * gather. Same sums, bit for bit. No reference counterpart. */
-#define GRG_HIP_FLAG_WAVE_KEYS 4096
+/* Per-row light path with one row per lane group (rows_per_group 1; merged
+ * entries or one segment): the group loads each round's entry keys or source
+ * indices together (one dword per lane per kl * es entries) and hands them out
+ * with lane shuffles, instead of one load per lane and gather. Same sums, bit
+ * for bit. Not for tiled, panel, streamed or partials launches. No reference
+ * counterpart. */
+#define GRG_HIP_FLAG_GROUP_KEYS 4096
 /* Bytes of one staged row (kl * cpl columns) in a tiled launch, at most. */
 #define GRG_HIP_TILE_CHUNK_BYTES 128
 /* Staged rows of one tile, at most (uint16 LDS indices). */

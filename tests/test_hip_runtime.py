@@ -182,33 +182,39 @@ def test_xcd_runs_light_path(art_a, oracle_a, dtype, log_run, form):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("unroll", [4, 8, 16])
-def test_wave_loaded_keys_match_merged_path(art_a, oracle_a, dtype, unroll):
-    """Merged down levels with one row per wave, the keys loaded by the whole wave and
-    handed out by lane shuffles: bit for bit the sums of the per-lane key loads (same
-    order), exact against the oracle for +-1 inputs; k = 3 leaves lanes past the last
-    column, which load keys but gather nothing; split and heavy rows beside it."""
+@pytest.mark.parametrize("es", [1, 4, 64])
+def test_group_loaded_entries_match_per_lane_loads(art_a, oracle_a, dtype, unroll, es):
+    """One row per lane group with the group's entry keys (merged down levels) or source
+    indices (one-segment up levels and write-outs) loaded by the whole group and handed
+    out by lane shuffles: bit for bit the sums of the per-lane loads (same order), exact
+    against the oracle for +-1 inputs. es = 64 is one row per wave; k = 3 leaves lanes
+    past the last column, which load entries but gather nothing; split and heavy rows
+    beside it."""
     from pygrgl_spmv.backends.hip import HipRuntime, ffi
 
     layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
-    base = {"unroll": unroll, "rows_per_group": 1, "es": 64, "flags": ffi.FLAG_MERGED}
-    wave = dict(base, flags=ffi.FLAG_MERGED | ffi.FLAG_WAVE_KEYS)
-    rng = np.random.default_rng(3450 + unroll)
     outs = {}
-    for name, shape in (("merged", base), ("wave", wave)):
-        with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16) as rt:
-            (grg,) = rt.grgs
-            r = np.random.default_rng(3460 + unroll)
-            res = []
-            for k in (1, 3, 16):
-                y = r.standard_normal((k, oracle_a.num_mutations)).astype(dtype)
-                m = r.standard_normal((k, oracle_a.num_mutations)).astype(dtype)
-                res.append(grg.matmul(y, "down", miss=m))
-                res.append(grg.matmul(y, "down", by_individual=True))
-            t = rng.choice(np.array([-1.0, 1.0]), size=(8, oracle_a.num_mutations))
-            np.testing.assert_array_equal(grg.matmul(t.astype(dtype), "down"), O.matmul(oracle_a, t, "down"))
-            outs[name] = res
-    for a, b in zip(outs["merged"], outs["wave"]):
-        np.testing.assert_array_equal(a, b)
+    for name, extra in (("per-lane", 0), ("group", ffi.FLAG_GROUP_KEYS)):
+        for form, fl in (("merged", ffi.FLAG_MERGED), ("plain", 0)):
+            shape = {"unroll": unroll, "rows_per_group": 1, "es": es, "flags": fl | extra}
+            with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16) as rt:
+                (grg,) = rt.grgs
+                r = np.random.default_rng(3460 + unroll + es)
+                res = []
+                for k in (1, 3, 16):
+                    y = r.standard_normal((k, oracle_a.num_mutations)).astype(dtype)
+                    m = r.standard_normal((k, oracle_a.num_mutations)).astype(dtype)
+                    x = r.standard_normal((k, oracle_a.num_samples)).astype(dtype)
+                    res.append(grg.matmul(y, "down", miss=m))
+                    res.append(grg.matmul(y, "down", by_individual=True))
+                    res.append(grg.matmul(x, "up"))
+                for direction, cols in (("up", oracle_a.num_samples), ("down", oracle_a.num_mutations)):
+                    t = r.choice(np.array([-1.0, 1.0]), size=(8, cols))
+                    np.testing.assert_array_equal(grg.matmul(t.astype(dtype), direction), O.matmul(oracle_a, t, direction))
+                outs[(name, form)] = res
+    for form in ("merged", "plain"):
+        for a, b in zip(outs[("per-lane", form)], outs[("group", form)]):
+            np.testing.assert_array_equal(a, b)
 
 
 def test_xcd_runs_flags_are_validated():
