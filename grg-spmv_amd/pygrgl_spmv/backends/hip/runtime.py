@@ -78,7 +78,7 @@ PANEL_MAX_PASSES = 2048       # passes of one panel (their entry offsets sit in 
 DEGREE_WINDOW = 4096    # nodes per degree-sorted run of the device node order
 DEGREE_MIN_MEAN = 6.0
 # Run lengths (log2 of light blocks) the tuner tries for GRG_HIP_FLAG_XCD_RUNS.
-XCD_RUN_LOGS = (4, 6, 9, 12)
+XCD_RUN_LOGS = (4, 5, 6, 8, 9, 10, 12)
 # The tuner's last stage: the fastest few distinct launches re-timed, interleaved, in
 # this many rounds, for launches of at least this many milliseconds.
 TUNE_FINALISTS = 4
