@@ -122,15 +122,15 @@ def test_archive_is_uncompressed_with_direct_scan_keys(synth_artifact):
 def test_scan_does_not_load_structural_arrays(tmp_path, monkeypatch):
     """Reference test_artifacts.py:231-252: n=4, bw=1 band -> blocks [(1,0,4),(2,0,4),(2,1,4)]."""
     art = write_band_artifact(tmp_path, "direct-scan", n=4, bandwidth=1)
-    real = artifact_module._load_struct_array
+    real = artifact_module._read_struct
 
-    def guarded(data, key, *, non_negative=True):
+    def guarded(npz, key):
         if key != "level_offsets":
             raise AssertionError(f"scan loaded structural array {key}")
-        return real(data, key, non_negative=non_negative)
+        return real(npz, key)
 
-    artifact_module._scan_grg_spmv_cached.cache_clear()
-    monkeypatch.setattr(artifact_module, "_load_struct_array", guarded)
+    artifact_module.clear_scan_cache()
+    monkeypatch.setattr(artifact_module, "_read_struct", guarded)
     scan = scan_grg_spmv(art)
     assert scan.num_levels == 3
     assert scan.selector_mut_nnz == 4 and scan.selector_miss_nnz == 0
@@ -179,3 +179,55 @@ def test_generated_artifact_invariants(synth_artifact):
         reach = np.zeros(off[d + 1] - off[d], dtype=bool)
         reach |= np.diff(state.A_blocks[d][d - 1].indptr) > 0
         assert reach.all()
+
+
+def test_written_keys_and_dtypes_follow_format_8(synth_artifact):
+    """Every key of format 8 (reference artifact.py:124-184) with its storage dtype; nothing else."""
+    with np.load(synth_artifact, allow_pickle=False) as z:
+        files = set(z.files)
+        got = {k: z[k].dtype for k in files}
+        levels = z["scan_block_levels"]
+    counts = ("num_samples", "num_mutations", "num_nodes", "ploidy", "num_individuals", "num_edges")
+    flags = ("has_missing_data", "has_individual_coals", "has_xtx_bias")
+    structs = ("level_offsets", "node_perm", "sample_to_individual", "sel_mut_indices", "sel_mut_indptr",
+               "sel_miss_indices", "sel_miss_indptr")
+    fixed = {"grg_spmv_format_version": np.int32, "mutation_positions": np.float64, "mutation_times": np.float64,
+             "coalescence_counts": np.int64, "scan_block_levels": np.int32, "scan_block_shapes": np.int64,
+             "scan_block_nnzs": np.int64, "scan_block_struct_itemsize": np.uint8, "scan_selector_nnzs": np.int64,
+             "scan_selector_nnz_by_level": np.int64, "scan_metadata_struct_itemsize": np.uint8}
+    free = ("grg_spmv_magic", "mutation_alleles", "mutation_allele_offsets", "mutation_ref_alleles",
+            "mutation_ref_allele_offsets", "init_vector_up_bias", "init_vector_down_bias", "init_xtx_up_bias",
+            "init_xtx_down_bias")
+    blocks = {f"A_blocks_{d}_{s}_{part}" for d, s in levels.tolist() for part in ("indices", "indptr")}
+    want = set(counts) | set(flags) | set(structs) | set(fixed) | set(free) | blocks
+    assert files == want
+    for k in counts:
+        assert got[k] == np.int64
+    for k in flags:
+        assert got[k] == np.bool_
+    for k in structs + tuple(blocks):
+        assert got[k] in (np.int32, np.int64)
+    for k, dt in fixed.items():
+        assert got[k] == dt, k
+    # a block for every level pair d > s, empty ones included
+    state = load_grg_spmv(synth_artifact, np.float64)
+    L = len(state.level_offsets) - 1
+    assert sorted(map(tuple, levels.tolist())) == [(d, s) for d in range(L) for s in range(d)]
+
+
+def test_presence_flags_follow_optional_keys(tmp_path, synth_artifact):
+    """Dropping the coalescence counts and the XTX biases clears both flags, and the loader
+    then returns None for them."""
+    state = load_grg_spmv(synth_artifact, np.float64)
+    assert state.coalescence_counts is not None and state.init_xtx_up_bias is not None
+    state.coalescence_counts = None
+    state.init_xtx_down_bias = None
+    out = tmp_path / "no-optional.grg_spmv"
+    save_grg_spmv(state, out)
+    with np.load(out, allow_pickle=False) as z:
+        assert not bool(z["has_individual_coals"]) and not bool(z["has_xtx_bias"])
+        assert "coalescence_counts" not in z.files and "init_xtx_down_bias" not in z.files
+    back = load_grg_spmv(out, np.float64)
+    assert back.coalescence_counts is None and back.init_xtx_up_bias is None and back.init_xtx_down_bias is None
+    scan = scan_grg_spmv(out)
+    assert not scan.has_individual_coals and not scan.has_xtx_bias
