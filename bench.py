@@ -67,6 +67,20 @@ def cpu_threads() -> int:
     return max(1, n)
 
 
+def host_cores() -> dict:
+    """Core counts of this host: logical (``os.cpu_count()``, what the reference's MklRuntime uses
+    for ``n_threads=0``, ``mkl/backend.py:296-300``), physical (psutil), and the cores this
+    process may run on (its affinity mask)."""
+    try:
+        import psutil
+
+        physical = psutil.cpu_count(logical=False)
+    except Exception:  # psutil absent or the count unknown
+        physical = None
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"logical_cores": os.cpu_count(), "physical_cores": physical, "affinity_cores": affinity}
+
+
 def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss: bool, calibrate_mib: int = 0,
             runtime_kwargs: dict | None = None):
     import torch
@@ -187,7 +201,7 @@ def cpu_baseline(artifact: Path, stats: dict, *, k: int, use_miss: bool, budget_
                   f"whole workload, mean of {len(times)} up+down steps after 1 warmup; seeding and selector gathers "
                   f"outside the timed sweep; {sweep.n_threads} threads; library {sweep.path}",
         "ms_per_step": per_step * 1e3,
-        "cpu_count": os.cpu_count(),
+        **host_cores(),
     }
 
 

@@ -34,6 +34,7 @@ from __future__ import annotations
 
 from contextlib import contextmanager
 import ctypes
+import hashlib
 from dataclasses import dataclass, field
 from pathlib import Path
 
@@ -41,7 +42,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from pygrgl_spmv.backends.base import copy_struct_checked, relink_stream_dependencies
-from pygrgl_spmv.backends.hip import ffi
+from pygrgl_spmv.backends.hip import ffi, tunecache
 from pygrgl_spmv.backends.hip.layout import PANEL_ROWS, PANEL_STAGED, SPLIT_CHUNK, SPLIT_MIN, HipLayout, enabled_max_k, side_io_rows
 from pygrgl_spmv.backends.types import Direction, InitMode
 from pygrgl_spmv.grg import BoundGRG, _CudaMatmulSpec
@@ -315,6 +316,7 @@ class _HipPreparedMatmul:
             self._init_matrix_internal = runtime._io1[: st.num_nodes, :k]
             self.init_matrix = self._init_matrix_internal.T
         self._steps = runtime._build_steps(artifact, spec)
+        runtime._flush_tune_cache()
 
     def __enter__(self) -> "_HipPreparedMatmul":
         return self
@@ -421,6 +423,7 @@ class HipBatchedSweep:
                                 aux=torch.zeros((st.num_nodes, k), dtype=tdt, device=dev) if spec.emit_all_nodes else None,
                                 partials=torch.zeros((items_max * k,), dtype=torch.float64, device=dev))
                 steps = runtime._build_steps(art, spec, bufs)
+                runtime._flush_tune_cache()
                 if any(it[0] != "batch" for it in steps):
                     raise ValueError("a batched sweep needs every block of its GRG resident")
                 p = 0
@@ -484,7 +487,7 @@ class HipRuntime:
                  tile_max_staged: int = TILE_MAX_STAGED, tile_max_rows: int = TILE_MAX_ROWS,
                  tile_max_entries: int = TILE_MAX_ENTRIES, degree_window: int = DEGREE_WINDOW,
                  degree_min_mean: float = DEGREE_MIN_MEAN, panels: bool = False, panel_min_mean: float = PANEL_MIN_MEAN,
-                 panel_min_rows: int = PANEL_MIN_ROWS) -> None:
+                 panel_min_rows: int = PANEL_MIN_ROWS, tune_cache="default") -> None:
         """``autotune``: time the admissible launch shapes of every level launch when an
         op is prepared (otherwise a fixed default shape). ``split_min`` /
         ``split_chunk``: rows of gather degree above ``split_min`` are summed in
@@ -504,7 +507,9 @@ class HipRuntime:
         on MI355X the per-row forms are faster at every level measured so far): with the
         tile tables, also build the panels of the panel launches (levels whose light rows
         average at least ``panel_min_mean`` entries, with at least ``panel_min_rows``
-        rows) and let the tuner choose them."""
+        rows) and let the tuner choose them. ``tune_cache``: directory of the on-disk cache of
+        tuned launch shapes (``tunecache``; "default": ``$GRG_HIP_TUNE_CACHE`` or
+        ``~/.cache/pygrgl_spmv/hip_tune``; None: no disk cache)."""
         if int(degree_window) < 0:
             raise ValueError("degree_window must be >= 0")
         if int(split_min) < 1 or int(split_chunk) < 1:
@@ -535,6 +540,8 @@ class HipRuntime:
         self.degree_window = int(degree_window)
         self.degree_min_mean = float(degree_min_mean)
         self._dtype_code = ffi.F64 if np.dtype(layout.dtype) == np.float64 else ffi.F32C
+        self._tune_dir = tunecache.default_directory() if tune_cache == "default" else (
+            None if tune_cache is None else Path(tune_cache))
         self._reset_owned_state()
 
     # ---- lifecycle -------------------------------------------------------
@@ -557,6 +564,7 @@ class HipRuntime:
         self._artifacts = ()
         self._grgs = ()
         self._tune_cache = {}
+        self._tune_ctx = {}
         self.tune_log = {}
         self._sweep_timer = None
         self._entered = False
@@ -606,6 +614,9 @@ class HipRuntime:
             for ai, (al, st) in enumerate(zip(lay.artifacts, states)):
                 arts.append(self._build_artifact(ai, al, st))
             self._artifacts = tuple(arts)
+            if self.autotune and self._tune_dir is not None:
+                for al, art in zip(lay.artifacts, arts):
+                    self._tune_ctx[art.path] = tunecache.TuneContext(self._tune_dir, self._tune_context_fields(al, art))
             items = max([a.max_items for a in arts] + [1])
             with torch.cuda.device(self.device):
                 # FP64 partial rows of split rows, reused by every level launch in stream order
@@ -919,6 +930,39 @@ class HipRuntime:
 
     # ---- launch construction ----------------------------------------------
 
+    def _tune_context_fields(self, al, art: _HipArtifact) -> dict:
+        """What the tuned shapes of one artifact depend on besides the launch itself."""
+        torch = _torch()
+        lib = Path(ffi.LIB_PATH).stat()
+        props = torch.cuda.get_device_properties(self.device)
+        resident = "".join("1" if b.resident else "0" for b in (*al.blocks_up, *al.blocks_down))
+        return {"artifact": tunecache.file_digest(art.path), "dtype": int(self._dtype_code), "abi": ffi.ABI_VERSION,
+                "library": [int(lib.st_size), int(lib.st_mtime_ns)],
+                "device": [str(props.name), str(getattr(props, "gcnArchName", "")), int(props.multi_processor_count)],
+                "split": [self.split_min, self.split_chunk], "heavy": int(self.layout.heavy_threshold),
+                "tiles": [bool(self.layout.tiles and self.use_tiles), self.tile_max_staged, self.tile_max_rows,
+                          self.tile_max_entries],
+                "panels": [self.use_panels, self.panel_min_mean, self.panel_min_rows],
+                "order": [self.degree_window, self.degree_min_mean],
+                "resident": hashlib.blake2b(resident.encode(), digest_size=8).hexdigest()}
+
+    def _flush_tune_cache(self) -> None:
+        for ctx in self._tune_ctx.values():
+            ctx.flush()
+
+    def _shape_admissible(self, a: ffi.GrgHipGatherArgs, shape, lds) -> bool:
+        """Whether a cached shape is one the kernel accepts for this launch."""
+        cpl, kl, es, unroll, rows, fl = shape
+        if (cpl, kl) not in column_shapes(a.dtype, a.k, lds) and not (fl & (ffi.FLAG_TILED | ffi.FLAG_PANEL)):
+            return False
+        if cpl < 1 or a.k % cpl or any(int(ld) % cpl for ld in lds) or kl < 1 or kl & (kl - 1) or es < 1 or es & (es - 1):
+            return False
+        if (fl & ffi.FLAG_TILED and not a.tile_row) or (fl & ffi.FLAG_PANEL and not a.pan_pass):
+            return False
+        if fl & ffi.FLAG_MERGED and not a.ekey:
+            return False
+        return (unroll, rows) in ((4, 1), (8, 1), (16, 1), (4, 2), (8, 2), (4, 4))
+
     def _elem(self) -> int:
         return 8 if self._dtype_code == ffi.F64 else 4
 
@@ -1081,9 +1125,21 @@ class HipRuntime:
         and the tiled form. The fastest launch is cached per artifact, sweep
         configuration, level and k."""
         torch = _torch()
+        lds = [a.ld_out] + [a.seg[i].ld for i in range(a.nseg)] + [a.map[i].ld for i in range(a.nmap)]
+        # the leading dimensions decide which columns-per-lane widths are admissible, so a
+        # shape tuned for one set of buffers (the runtime's arena at ld = max_k, a batched
+        # sweep's private buffers at ld = k) is only reused for launches with the same ones
+        key = tuple(key) + (tuple(int(v) for v in lds),)
         best = self._tune_cache.get(key)
+        ctx = self._tune_ctx.get(key[0]) if self.autotune else None
+        disk_key = tunecache.launch_key(key[1], key[2], key[3], lds) if ctx is not None else None
+        if best is None and ctx is not None and a.row_end > a.row_begin:
+            cached = ctx.get(disk_key)
+            if cached is not None and self._shape_admissible(a, cached, lds):
+                best = cached
+                self._tune_cache[key] = best
+                self.tune_log[key] = {"rows": int(a.row_end - a.row_begin), "best": (None,) + best, "cached": True}
         if best is None and self.autotune and a.row_end > a.row_begin:
-            lds = [a.ld_out] + [a.seg[i].ld for i in range(a.nseg)] + [a.map[i].ld for i in range(a.nmap)]
             trial = ffi.GrgHipGatherArgs()
             ctypes.pointer(trial)[0] = a
             elem = 8 if a.dtype == ffi.F64 else 4
@@ -1186,6 +1242,8 @@ class HipRuntime:
                     top = (again[winner],) + winner
             best = top[1:]
             self._tune_cache[key] = best
+            if ctx is not None:
+                ctx.put(disk_key, best)
             tiled = [t for t in timings if t[6] & ffi.FLAG_TILED]
             untiled = [t for t in timings if not t[6] & ffi.FLAG_TILED]
             self.tune_log[key] = {"rows": int(a.row_end - a.row_begin), "best": top,

@@ -38,7 +38,8 @@ def mixed_k(i: int) -> int:
     return (1, 3, 8, 16, 5, 32, 64, 2, 12, 48, 7)[i % 11]
 
 
-def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: int = 3, seed: int = 4) -> dict:
+def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: int = 3, seed: int = 4,
+        autotune: bool = True) -> dict:
     import torch
 
     from pygrgl_spmv.backends.hip import HipPlanPair, HipRuntime, plan_hip_layout
@@ -59,7 +60,7 @@ def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: i
     log(f"layout: {out['bytes_total'] / 1e9:.2f} GB of {budget_bytes / 1e9:.1f} GB budget, "
         f"{out['streamed_blocks']} of {out['blocks']} blocks streamed")
     rng = np.random.default_rng(seed)
-    with HipRuntime(layout) as rt:
+    with HipRuntime(layout, autotune=autotune) as rt:
         grgs = rt.grgs
         ops, xs, ys, res_up, res_dn = [], [], [], [], []
         edges = 0

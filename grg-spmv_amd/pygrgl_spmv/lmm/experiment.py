@@ -62,7 +62,7 @@ def run(artifacts, *, k: int = 64, r: int = 20, n_train: int = 2048, n_test: int
     from pygrgl_spmv.grg import RuntimeRequirements
     from pygrgl_spmv.lmm.operator import StandardizedGrgOperator
     from pygrgl_spmv.lmm.pcs import top_eigenpairs
-    from pygrgl_spmv.lmm.simulate import simulate
+    from pygrgl_spmv.lmm.simulate import Phenotypes, simulate
     from pygrgl_spmv.lmm.warmstart import Basis, WarmStartNet, closed_form_coefficients, start_from_coefficients, targets
 
     req = RuntimeRequirements(max_k_up=k, max_k_down=k, need_down_miss_input=False, need_up_miss_output=False,
@@ -94,19 +94,32 @@ def run(artifacts, *, k: int = 64, r: int = 20, n_train: int = 2048, n_test: int
         log(f"{n_train} training solves in {out['train_solve_seconds']:.1f}s, mean {its_train.mean():.1f} iterations")
 
         t = time.perf_counter()
+        # the last n_val training solves are held out: they decide whether the network is deployed
+        n_val = max(k, (n_train // 8) // k * k)
+        n_fit = n_train - n_val
         net = WarmStartNet(basis, seed=seed)
-        history = net.fit(train.y, train.delta, x_train, epochs=epochs)
+        history = net.fit(train.y[:, :n_fit], train.delta[:n_fit], x_train[:, :n_fit], epochs=epochs)
         torch.cuda.synchronize()
         out["train_seconds"] = time.perf_counter() - t
         out["train_loss_first_last"] = [history[0], history[-1]]
-        log(f"network trained in {out['train_seconds']:.1f}s, loss {history[0]:.4f} -> {history[-1]:.4f}")
+        log(f"network trained on {n_fit} solves in {out['train_seconds']:.1f}s, loss {history[0]:.4f} -> {history[-1]:.4f}")
+        val = Phenotypes(y=train.y[:, n_fit:], h2=train.h2[n_fit:], delta=train.delta[n_fit:])
+        _, its_net, _, _, _ = solve_batches(op, val, rel_tol=rel_tol, max_iter=max_iter,
+                                            start=lambda y, d, s: net.network_start(y, d))
+        _, its_cf, _, _, _ = solve_batches(op, val, rel_tol=rel_tol, max_iter=max_iter, start=lambda y, d, s:
+                                           start_from_coefficients(basis, y, closed_form_coefficients(basis, y, d)))
+        net.choose(float(its_net.mean()), float(its_cf.mean()))
+        out["validation"] = dict(net.validation, phenotypes=int(n_val))
+        log(f"validation on {n_val} held-out solves: network {its_net.mean():.2f}, closed form {its_cf.mean():.2f} "
+            f"mean iterations; network deployed: {net.use_network}")
 
         test = simulate(op, n_test, seed=seed + 2)
         results = {}
         starts = {
             "zero": None,
             "closed_form": lambda y, d, s: start_from_coefficients(basis, y, closed_form_coefficients(basis, y, d)),
-            "network": lambda y, d, s: net.start(y, d),
+            "network": lambda y, d, s: net.start(y, d),        # as deployed (the closed form if validation rejected it)
+            "network_raw": lambda y, d, s: net.network_start(y, d),
             # the least-squares best start of the same form, fitted to the zero-start solution:
             # the bound any predictor of those coefficients can reach on this data
             "best_in_span": lambda y, d, s: start_from_coefficients(
@@ -123,7 +136,7 @@ def run(artifacts, *, k: int = 64, r: int = 20, n_train: int = 2048, n_test: int
                              "max_rel_residual": float(rels.max()), "seconds": time.perf_counter() - t,
                              "unconverged": int((its < 0).sum())}
             log(f"test {name}: mean {its.mean():.1f} iterations, {apps} operator applications")
-        for name in ("closed_form", "network", "best_in_span"):
+        for name in ("closed_form", "network", "network_raw", "best_in_span"):
             diff = torch.linalg.norm(xs[name] - xs["zero"], dim=0) / torch.linalg.norm(xs["zero"], dim=0)
             results[name]["max_rel_diff_vs_zero_start"] = float(diff.max())
             results[name]["iterations_saved"] = 1.0 - results[name]["mean_iterations"] / results["zero"]["mean_iterations"]

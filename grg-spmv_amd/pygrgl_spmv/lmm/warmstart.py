@@ -96,15 +96,30 @@ def closed_form_coefficients(basis: Basis, y, delta):
 class WarmStartNet:
     """The network predicts a correction to the closed-form coefficients: its output
     (de-standardised with training-set statistics) is added to
-    ``closed_form_coefficients``, so an untrained or useless network falls back to
-    the closed-form start. ``DenseNet([r + 1, *hidden, r + 1])``, trained with Adam
-    on the GPU."""
+    ``closed_form_coefficients``. ``DenseNet([r + 1, *hidden, r + 1])``, trained with
+    Adam on the GPU.
+
+    The network is only deployed when it earns it: ``choose`` takes the mean CG
+    iterations of held-out validation solves from both starts, and ``start`` uses
+    the network only if it needed fewer iterations there by at least ``margin``
+    (otherwise the closed-form start)."""
 
     def __init__(self, basis: Basis, *, hidden=(256, 256), seed: int = 0):
         self.basis = basis
         r = int(basis.U.shape[1])
         self.net = DenseNet([r + 1, *hidden, r + 1], device=basis.U.device, seed=seed)
         self.stats = None
+        self.use_network = True
+        self.validation = None
+
+    def choose(self, network_iterations: float, closed_form_iterations: float, *, margin: float = 0.5) -> bool:
+        """Deploy the network only if its validation starts needed fewer CG iterations than the
+        closed-form starts by at least ``margin`` (mean per phenotype). Returns the choice."""
+        self.validation = {"network_mean_iterations": float(network_iterations),
+                           "closed_form_mean_iterations": float(closed_form_iterations), "margin": float(margin)}
+        self.use_network = float(network_iterations) <= float(closed_form_iterations) - float(margin)
+        self.validation["network_deployed"] = self.use_network
+        return self.use_network
 
     def fit(self, y, delta, x_star, *, epochs: int = 300, batch: int = 128, lr: float = 1e-3, seed: int = 0):
         """Train on solutions ``x_star`` of the plain method for phenotypes ``y`` (n x c)
@@ -126,8 +141,14 @@ class WarmStartNet:
         correction = self.net.forward((F - fm) / fs) * ts + tm
         return closed_form_coefficients(self.basis, y, delta) + correction.to(torch.float64)
 
-    def start(self, y, delta):
+    def network_start(self, y, delta):
         return start_from_coefficients(self.basis, y, self.coefficients(y, delta))
+
+    def start(self, y, delta):
+        """The deployed start: the network's, or the closed form when validation rejected it."""
+        if not self.use_network:
+            return start_from_coefficients(self.basis, y, closed_form_coefficients(self.basis, y, delta))
+        return self.network_start(y, delta)
 
 
 __all__ = ["Basis", "WarmStartNet", "closed_form_coefficients", "features", "start_from_coefficients", "targets"]

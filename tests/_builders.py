@@ -107,3 +107,39 @@ def write_synthetic_artifact(artifact_dir, name, cfg) -> Path:
     if not path.exists():
         write_artifact(cfg, path)
     return path
+
+
+def bench_dir() -> Path:
+    """Where the benchmark workloads' artifacts are cached (shared with ``bench.py``)."""
+    import os
+
+    d = Path(os.environ.get("GRG_BENCH_DIR", "/tmp/grg_bench"))
+    d.mkdir(parents=True, exist_ok=True)
+    return d
+
+
+def workload_artifact(name: str) -> Path:
+    """The named synthetic workload (``pygrgl_spmv.synth.WORKLOADS``), generated once from its seed."""
+    from pygrgl_spmv.synth import WORKLOADS, write_artifact
+
+    cfg = WORKLOADS[name]
+    p = bench_dir() / f"{name}-seed{cfg.seed}.grg_spmv"
+    if not p.exists():
+        write_artifact(cfg, p)
+    return p
+
+
+def chromosome_artifacts(count: int = 22, samples: int = 200_000, workers: int = 8) -> list[Path]:
+    """The per-chromosome workload (``synth.chromosome_workloads``), generated in parallel processes."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from pygrgl_spmv.synth import chromosome_workloads, write_artifact
+
+    d = bench_dir()
+    cfgs = chromosome_workloads(count, samples)
+    paths = [d / f"chr{i + 1}-{samples}-seed{c.seed}.grg_spmv" for i, c in enumerate(cfgs)]
+    todo = [(c, p) for c, p in zip(cfgs, paths) if not p.exists()]
+    if todo:
+        with ProcessPoolExecutor(max(1, min(int(workers), len(todo)))) as ex:
+            list(ex.map(write_artifact, [c for c, _ in todo], [p for _, p in todo]))
+    return paths

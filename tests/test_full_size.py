@@ -7,7 +7,7 @@
 * 500,000 samples x 5,000,000 mutations with 5 % missing genotypes (about
   271 M edges): size-independent properties -- all-ones inputs give the MKL
   sweep's integer counts exactly, adjointness <y, X^T x> = <X y, x> holds to
-  1e-12, and the FP32-compensated sweeps stay within 1e-5 of FP64 per column.
+  1e-12, and the FP32-compensated sweeps stay within 5e-7 of FP64 per column.
 
 The artifacts are generated from fixed seeds into ``GRG_BENCH_DIR`` (shared
 with ``bench.py``, so a later bench run reuses them).
@@ -112,4 +112,4 @@ def test_c3_properties_at_full_size():
     up32, dn32 = out["float32"]
     err = max(_col_err(up32, up64), _col_err(dn32, dn64))
     print(f"c3: FP32c max per-column relative error vs FP64 {err:.2e}", flush=True)
-    assert err <= 1e-5
+    assert err <= 5e-7

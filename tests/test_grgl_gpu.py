@@ -169,7 +169,7 @@ def test_streamed_layouts_on_fixture(fixtures, mode):
 @pytest.mark.parametrize("name", ["t200", "msprime"])
 def test_fp32c_on_fixture(fixtures, name):
     """The FP32-compensated form on the reference's GRGs: exact for +-1 inputs (integer
-    sums below 2^24), within 1e-5 normwise for random inputs, missing data included."""
+    sums below 2^24), within 5e-7 normwise for random inputs, missing data included."""
     from pygrgl_spmv.backends.hip import HipRuntime
 
     path, g, X, Miss = fixtures[name]
@@ -183,7 +183,7 @@ def test_fp32c_on_fixture(fixtures, name):
         np.testing.assert_array_equal(grg.matmul(t.astype(np.float32), "down"), (t @ X.T).astype(np.float32))
         y, m = rng.standard_normal((4, g.num_mutations)), rng.standard_normal((4, g.num_mutations))
         got = grg.matmul(y.astype(np.float32), "down", miss=m.astype(np.float32))
-        assert O.normwise_rel_err(got, y @ X.T + m @ Miss.T) <= 1e-5
+        assert O.normwise_rel_err(got, y @ X.T + m @ Miss.T) <= 5e-7
         x = rng.standard_normal((4, g.num_individuals))
         got = grg.matmul(x.astype(np.float32), "up", by_individual=True)
-        assert O.normwise_rel_err(got, x @ _diploid(X)) <= 1e-5
+        assert O.normwise_rel_err(got, x @ _diploid(X)) <= 5e-7

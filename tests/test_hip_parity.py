@@ -3,7 +3,8 @@
 Bar: FP64 results match the oracle within 1e-12 normwise relative error
 (per output column, max|a - b| / max|b|) and exactly for integer-valued
 inputs (+-1, all-ones, 0/1: sums of integers below 2^53 are order-independent);
-FP32 (compensated) results within 1e-5 normwise relative error.
+FP32 (compensated) results within 5e-7 normwise relative error (the FP32 storage
+floor is 2^-24 per stored node value; measured errors are 1e-7 to 3e-7).
 Mode matrix follows the reference's ``runtime/test_prepare_cuda.py:124-148``
 (direction x by_individual x init mode), plus missing data, emit_all_nodes,
 a k sweep like ``runtime/test_traversal_correctness.py:14,59-74`` and the band
@@ -21,7 +22,7 @@ from tests._builders import build_hip_layout, full_requirements, write_band_arti
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12
-FP32_TOL = 1e-5
+FP32_TOL = 5e-7
 
 
 def _cfg_small():

@@ -21,6 +21,7 @@ from tests._builders import build_hip_layout, full_requirements, write_band_arti
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12
+FP32_TOL = 5e-7   # FP32-compensated sweeps: the FP32 storage floor (measured 1e-7 to 3e-7)
 
 
 def _cfg(seed, samples=300, mutations=2000, ploidy=2, missing=0.05):
@@ -121,13 +122,13 @@ def test_streamed_light_path(art_a, oracle_a, dtype, unroll, es):
     """The software-pipelined light path (waves looping over row batches, small and
     capped grids) on every level and write-out launch it applies to, with merged
     entry keys for the multi-segment down levels and heavy / split rows beside it:
-    exact for +-1 inputs, FP64 within 1e-12, FP32c within 1e-5."""
+    exact for +-1 inputs, FP64 within 1e-12, FP32c within 5e-7."""
     from pygrgl_spmv.backends.hip import HipRuntime
     from pygrgl_spmv.backends.hip import ffi
 
     layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
     shape = {"unroll": unroll, "rows_per_group": 1, "es": es, "flags": ffi.FLAG_STREAM}
-    tol = FP64_TOL if dtype == np.float64 else 1e-5
+    tol = FP64_TOL if dtype == np.float64 else FP32_TOL
     with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16) as rt:
         (grg,) = rt.grgs
         rng = np.random.default_rng(3377 + unroll + es)
@@ -159,14 +160,14 @@ def test_xcd_runs_light_path(art_a, oracle_a, dtype, log_run, form):
     window, here all of them for 2^12, keep their order) on every per-row light path
     (segment by segment or merged) or tiled launch of both sweeps, with heavy and
     split rows beside it: every row produced once, exact for +-1 inputs, FP64 within
-    1e-12, FP32c within 1e-5."""
+    1e-12, FP32c within 5e-7."""
     from pygrgl_spmv.backends.hip import HipRuntime, ffi
     from pygrgl_spmv.backends.hip.runtime import xcd_runs_flags
 
     layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
     flags = xcd_runs_flags(log_run) | {"light": 0, "merged": ffi.FLAG_MERGED, "tiled": ffi.FLAG_TILED}[form]
     shape = {"unroll": 8, "rows_per_group": 1, "es": 2, "flags": flags}
-    tol = FP64_TOL if dtype == np.float64 else 1e-5
+    tol = FP64_TOL if dtype == np.float64 else FP32_TOL
     with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16) as rt:
         (grg,) = rt.grgs
         rng = np.random.default_rng(3390 + log_run)
@@ -232,7 +233,7 @@ def test_panel_launch(art_a, oracle_a, dtype, remap, kl):
     """The panel form (rows of a panel share passes of LDS-staged sources) on every
     level of both sweeps, with heavy and split rows beside it, several passes per
     panel (few staged sources per pass at small sizes) and one or several column
-    chunks: exact for +-1 inputs, FP64 within 1e-12, FP32c within 1e-5."""
+    chunks: exact for +-1 inputs, FP64 within 1e-12, FP32c within 5e-7."""
     from pygrgl_spmv.backends.hip import HipRuntime
     from pygrgl_spmv.backends.hip import ffi
 
@@ -240,7 +241,7 @@ def test_panel_launch(art_a, oracle_a, dtype, remap, kl):
     shape = {"unroll": 8, "rows_per_group": 1, "flags": ffi.FLAG_PANEL | (ffi.FLAG_XCD_REMAP if remap else 0)}
     if kl is not None:
         shape["kl"] = kl
-    tol = FP64_TOL if dtype == np.float64 else 1e-5
+    tol = FP64_TOL if dtype == np.float64 else FP32_TOL
     with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16, panels=True, panel_min_mean=0.0,
                     panel_min_rows=1) as rt:
         (grg,) = rt.grgs
@@ -315,7 +316,7 @@ def test_fp32_wide_lanes(art_a, oracle_a, unroll, rows):
             x = rng.choice(np.array([-1.0, 1.0], dtype=np.float32), size=(16, _cols(oracle_a, direction)))
             np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x.astype(np.float64), direction))
             z = rng.standard_normal((16, _cols(oracle_a, direction))).astype(np.float32)
-            assert O.normwise_rel_err(grg.matmul(z, direction), O.matmul(oracle_a, z.astype(np.float64), direction)) <= 1e-5
+            assert O.normwise_rel_err(grg.matmul(z, direction), O.matmul(oracle_a, z.astype(np.float64), direction)) <= FP32_TOL
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -324,7 +325,7 @@ def test_fp32_wide_lanes(art_a, oracle_a, unroll, rows):
 def test_tiled_level_launches(art_a, oracle_a, dtype, es, remap, max_staged, max_rows):
     """Every level launch in its LDS-staged tiled form (small tiles, direct tiles for rows
     longer than max_staged, heavy and split rows beside the tiles), against the oracle in
-    every sweep configuration: exact for +-1 inputs, FP64 within 1e-12, FP32c within 1e-5."""
+    every sweep configuration: exact for +-1 inputs, FP64 within 1e-12, FP32c within 5e-7."""
     from pygrgl_spmv.backends.hip import HipRuntime
     from pygrgl_spmv.backends.hip import ffi
 
@@ -332,7 +333,7 @@ def test_tiled_level_launches(art_a, oracle_a, dtype, es, remap, max_staged, max
     assert layout.tiles and layout.bytes_by_category["tiles"] > 0
     flags = ffi.FLAG_TILED | (ffi.FLAG_XCD_REMAP if remap else 0)
     shape = {"es": es, "flags": flags}
-    tol = FP64_TOL if dtype == np.float64 else 1e-5
+    tol = FP64_TOL if dtype == np.float64 else FP32_TOL
     with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16, tile_max_staged=max_staged,
                     tile_max_rows=max_rows) as rt:
         (grg,) = rt.grgs
@@ -463,6 +464,42 @@ def test_streamed_ops_are_not_capturable(art_a):
             (grg,) = rt.grgs
             for direction in ("up", "down"):
                 assert grg.prepare_matmul_cuda(direction=direction, k=1).capturable is (not streamed)
+
+
+@pytest.mark.skip(reason="capturing a sweep with streamed blocks ended in a host segmentation fault inside "
+                         "hipStreamEndCapture (pinned-host slot copies on per-slot streams forked from the launch "
+                         "stream); no cause has been shown in this code, so the runtime refuses such a capture and "
+                         "this case stays off the GPU")
+def test_graph_capture_streamed_chain(art_a, oracle_a):
+    """Counterpart of the reference's streamed down -> up capture (``test_triton.py:945-990``):
+    every block streamed through a 2-slot ring, the chain captured once and replayed against the
+    oracle."""
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    with torch.cuda.device(0):
+        cap = torch.cuda.Stream()
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=1, max_k_down=1), stream=cap,
+                              ring_buffer_size=2, allow_residency=False)
+    order = ("down", "up")
+    src_np = np.arange(_cols(oracle_a, order[0]), dtype=np.float64).reshape(1, -1)
+    i1_np, i2_np = np.asarray([0.5]), np.asarray([1.5])
+    expected = _chain_expected(oracle_a, order, src_np, i1_np, i2_np)
+    with HipRuntime(layout) as rt, ExitStack() as stack:
+        (grg,) = rt.grgs
+        first, second = _enter_chain(stack, grg, order, 1)
+        dev = first.input.device
+        src = torch.from_numpy(src_np).to(dev)
+        i1, i2 = torch.from_numpy(i1_np).to(dev), torch.from_numpy(i2_np).to(dev)
+        result = torch.empty_like(second.output)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=cap):
+            _run_chain(first, second, src, i1, i2, result)
+        for rep in range(2):
+            graph.replay()
+            torch.cuda.synchronize()
+            _check(result.cpu().numpy(), expected)
 
 
 def test_graph_capture_chain_replays(art_a, oracle_a):
@@ -603,6 +640,50 @@ def test_batched_sweep_of_several_grgs(artifact_dir, art_a, art_b, persistent):
                     O.matmul(oa, inputs[6], "up", by_individual=True, init="xtx")]
             for out, w in zip(batch.outputs, want):
                 _check(out.cpu().numpy(), w)
+
+
+def test_tuned_shapes_follow_leading_dimensions(art_a, oracle_a):
+    """A batched sweep at k = 4 (private buffers with ld = 4, where 4 columns per lane are
+    admissible) is tuned first; an op of the same k on the runtime's arena (ld = max_k = 6,
+    so at most 2 columns per lane) must get a shape of its own, not the cached one."""
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=6, max_k_down=6))
+    rng = np.random.default_rng(3860)
+    with HipRuntime(layout) as rt:
+        (grg,) = rt.grgs
+        for direction in ("up", "down"):
+            batch = rt.prepare_batched_sweep([dict(grg=grg, direction=direction, k=4)], persistent=False)
+            x = rng.standard_normal((4, _cols(oracle_a, direction)))
+            batch.inputs[0].copy_(torch.from_numpy(x))
+            batch()
+            batch.check()
+            _check(batch.outputs[0].cpu().numpy(), O.matmul(oracle_a, x, direction))
+            _check(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
+
+
+def test_tuned_shapes_reused_from_disk(tmp_path, art_a, oracle_a):
+    """The first runtime tunes and writes its shapes; a second runtime over the same artifact
+    takes every level and write-out shape from the disk cache, with the same results."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=8, max_k_down=8))
+    rng = np.random.default_rng(3870)
+    x = rng.standard_normal((8, oracle_a.num_samples))
+    y = rng.standard_normal((8, oracle_a.num_mutations))
+    logs = []
+    for _ in range(2):
+        with HipRuntime(layout, tune_cache=tmp_path) as rt:
+            (grg,) = rt.grgs
+            _check(grg.matmul(x, "up"), O.matmul(oracle_a, x, "up"))
+            _check(grg.matmul(y, "down"), O.matmul(oracle_a, y, "down"))
+            logs.append(dict(rt.tune_log))
+    assert list(tmp_path.glob("*.json"))
+    assert logs[0] and not any(v.get("cached") for v in logs[0].values())
+    assert set(logs[1]) == set(logs[0]) and all(v.get("cached") for v in logs[1].values())
+    assert all(logs[1][key]["best"][1:] == logs[0][key]["best"][1:] for key in logs[0])
 
 
 def test_batched_sweep_captured_and_replayed(art_a, art_b):
