@@ -48,6 +48,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "../../include/grg_hip.h"
@@ -1671,6 +1672,43 @@ int launch_grid(const GrgHipGatherArgs& a, int64_t* gx, int64_t* gy, int64_t* n_
   return 0;
 }
 
+// Split-row partials of grg_hip_gather_batch_static: one array per GPU, held for
+// the life of the process. Its host-side name is only a handle: host code reaches
+// the device copy through the address hipGetSymbolAddress returns for the device
+// the stream belongs to (static_scratch below), never through the name itself.
+__device__ __align__(256) double scratch[GRG_HIP_STATIC_SCRATCH_DOUBLES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the stream's GPU's copy of `scratch`, looked up once per
+// device with that device current.
+int static_scratch(hipStream_t stream, double** out) {
+  int dev = 0;
+  hipError_t err = hipStreamGetDevice(stream, &dev);
+  if (err != hipSuccess) return fail(std::string("hipStreamGetDevice: ") + hipGetErrorString(err), 2);
+  if (dev < 0 || dev >= kMaxDevices) return fail("device ordinal " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    err = hipGetDevice(&cur);
+    if (err != hipSuccess) return fail(std::string("hipGetDevice: ") + hipGetErrorString(err), 2);
+    if (cur != dev && (err = hipSetDevice(dev)) != hipSuccess)
+      return fail(std::string("hipSetDevice: ") + hipGetErrorString(err), 2);
+    void* ptr = nullptr;
+    err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) {
+      hipError_t back = hipSetDevice(cur);
+      if (err == hipSuccess && back != hipSuccess) err = back;
+    }
+    if (err != hipSuccess || !ptr) return fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err), 2);
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = static_cast<double*>(g_scratch_ptr[dev]);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1777,6 +1815,30 @@ int grg_hip_gather_batch(const GrgHipGatherArgs* args, int32_t n, void* stream) 
   g_last_error.clear();
   for (int32_t i = 0; i < n; ++i) {
     if (int rc = launch_one(args[i], static_cast<hipStream_t>(stream))) {
+      g_last_error = "launch " + std::to_string(i) + ": " + g_last_error;
+      return rc;
+    }
+  }
+  return 0;
+}
+
+int grg_hip_gather_batch_static(const GrgHipGatherArgs* args, int32_t n, void* stream) {
+  if (n < 0 || (n > 0 && !args)) return fail("bad batch");
+  g_last_error.clear();
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double* part = nullptr;
+  if (int rc = static_scratch(s, &part)) return rc;
+  for (int32_t i = 0; i < n; ++i) {
+    GrgHipGatherArgs a = args[i];
+    if (a.split_ptr || (a.flags & GRG_HIP_FLAG_PARTIALS)) {
+      // the launch's split-row partials (split_items rows of k doubles) go to the static array
+      if (a.split_items < 0 || a.k < 1 || a.split_items > GRG_HIP_STATIC_SCRATCH_DOUBLES / a.k)
+        return fail("launch " + std::to_string(i) + ": split-row partials need " + std::to_string(a.split_items) + " x " +
+                    std::to_string(a.k) + " doubles, more than the static scratch holds (" +
+                    std::to_string((long long)GRG_HIP_STATIC_SCRATCH_DOUBLES) + " doubles)");
+      a.partials = part;
+    }
+    if (int rc = launch_one(a, s)) {
       g_last_error = "launch " + std::to_string(i) + ": " + g_last_error;
       return rc;
     }

@@ -13,7 +13,8 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 13
+ABI_VERSION = 14
+STATIC_SCRATCH_DOUBLES = 1 << 22
 F64 = 0
 F32C = 1
 FLAG_XCD_REMAP = 2
@@ -113,6 +114,8 @@ class HipLibrary:
         lib.grg_hip_gather.restype = ctypes.c_int
         lib.grg_hip_gather_batch.argtypes = [ctypes.POINTER(GrgHipGatherArgs), c_int32, c_void_p]
         lib.grg_hip_gather_batch.restype = ctypes.c_int
+        lib.grg_hip_gather_batch_static.argtypes = [ctypes.POINTER(GrgHipGatherArgs), c_int32, c_void_p]
+        lib.grg_hip_gather_batch_static.restype = ctypes.c_int
         lib.grg_hip_last_error.argtypes = []
         lib.grg_hip_last_error.restype = ctypes.c_char_p
         lib.grg_hip_abi_version.argtypes = []
@@ -150,6 +153,12 @@ class HipLibrary:
 
     def gather_batch(self, array, n: int, stream_ptr: int) -> None:
         self._check(self.lib.grg_hip_gather_batch(array, c_int32(int(n)), c_void_p(int(stream_ptr))), "grg_hip_gather_batch")
+
+    def gather_batch_static(self, array, n: int, stream_ptr: int) -> None:
+        """``grg_hip_gather_batch`` with the split-row partials in the library's static per-GPU
+        array instead of each descriptor's ``partials`` (no allocation on this path)."""
+        self._check(self.lib.grg_hip_gather_batch_static(array, c_int32(int(n)), c_void_p(int(stream_ptr))),
+                    "grg_hip_gather_batch_static")
 
     def jobs_build(self, args_list, phase_of):
         """Job table of launch descriptors (grg_hip_jobs_build): (jobs ctypes array,

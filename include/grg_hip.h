@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 13
+#define GRG_HIP_ABI_VERSION 14
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -226,6 +226,23 @@ int grg_hip_gather(const GrgHipGatherArgs* args, void* stream);
  * seed-fused levels then write-out, with the partials launch of a level's
  * split rows right before the level), from one host call. */
 int grg_hip_gather_batch(const GrgHipGatherArgs* args, int32_t n, void* stream);
+
+/* Doubles of the static split-row scratch of grg_hip_gather_batch_static (32 MiB
+ * per GPU). */
+#define GRG_HIP_STATIC_SCRATCH_DOUBLES (1LL << 22)
+
+/* grg_hip_gather_batch without a caller workspace: the split-row partials of
+ * every launch (args[i].partials is ignored) go to a file-scope device array of
+ * GRG_HIP_STATIC_SCRATCH_DOUBLES doubles, one per GPU, kept until the process
+ * ends; nothing is allocated on this path (no hipMalloc, hipFree or allocating
+ * library call). A launch whose partials need more than the array holds
+ * (split_items * k doubles) is rejected with an error before it runs. The
+ * partials launch of a level writes every partial row its level launch reads,
+ * so the array needs no clearing; calls that overlap on one GPU (other streams,
+ * other threads) share the array and race. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid allocation). */
+int grg_hip_gather_batch_static(const GrgHipGatherArgs* args, int32_t n, void* stream);
 
 /* ---- Job tables: many row-gather launches in one kernel -------------------
  *
