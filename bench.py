@@ -287,6 +287,8 @@ def main() -> int:
                     help="after the timed region, run one device copy of this many MiB (for PMC scaling)")
     ap.add_argument("--tune-log", default="",
                     help="write the per-level tuner choices and launch times of each form to this JSON file")
+    ap.add_argument("--runtime-option", action="append", default=[], metavar="NAME=VALUE",
+                    help="keyword argument of HipRuntime for both value types, e.g. panels=True (repeatable)")
     ap.add_argument("--gather-ceiling", action="store_true",
                     help="instead of the sweep: the sweep kernel's gather rate on synthetic random and windowed indices")
     args = ap.parse_args()
@@ -315,13 +317,21 @@ def main() -> int:
     use_miss = bool(scan.has_missing_data) and stats["sel_miss_nnz"] > 0
     log(f"workload {args.workload}: {stats} ready in {time.perf_counter() - t0:.1f}s; down sweep uses miss input: {use_miss}")
 
+    import ast
+
+    runtime_kwargs = {}
+    for opt in args.runtime_option:
+        name, _, value = opt.partition("=")
+        runtime_kwargs[name.strip()] = ast.literal_eval(value.strip())
+    if runtime_kwargs:
+        log(f"runtime options: {runtime_kwargs}")
     k = int(args.k)
     eu = 2 * stats["edges"] * k
     variants = {}
     f64 = None
     if not args.no_fp64:
         f64 = run_gpu(artifact, k=k, dtype=np.float64, steps=args.steps, warmup=args.warmup, use_miss=use_miss,
-                      calibrate_mib=args.calibrate_copy_mib)
+                      calibrate_mib=args.calibrate_copy_mib, runtime_kwargs=runtime_kwargs)
         t64 = f64["t_up"] + f64["t_down"]
         b64 = algorithmic_bytes(stats, k, 8, "up", False) + algorithmic_bytes(stats, k, 8, "down", use_miss)
         log(f"fp64: up {f64['t_up']*1e3:.3f} ms, down {f64['t_down']*1e3:.3f} ms, {eu / t64 / 1e9:.2f} G edge-updates/s, "
@@ -331,7 +341,7 @@ def main() -> int:
                             "wall_ms_per_step": f64["wall_per_step"] * 1e3}
     if not args.no_fp32c:
         f32 = run_gpu(artifact, k=k, dtype=np.float32, steps=args.steps, warmup=args.warmup, use_miss=use_miss,
-                      calibrate_mib=args.calibrate_copy_mib if args.no_fp64 else 0)
+                      calibrate_mib=args.calibrate_copy_mib if args.no_fp64 else 0, runtime_kwargs=runtime_kwargs)
         t32 = f32["t_up"] + f32["t_down"]
         b32 = algorithmic_bytes(stats, k, 4, "up", False) + algorithmic_bytes(stats, k, 4, "down", use_miss)
         v = {"value": eu / t32, "ms_per_step": t32 * 1e3, "ms_up": f32["t_up"] * 1e3, "ms_down": f32["t_down"] * 1e3,

@@ -93,6 +93,28 @@ __device__ __forceinline__ void store_raw(T* p, const T (&v)[CPL]) {
   }
 }
 
+// A device pointer to global memory, marked as such: loads through it become
+// global (not flat) instructions, which count against the vector-memory wait
+// counter only. Flat loads also count against the LDS counter, so a kernel that
+// mixes LDS reads with flat loads waits for every load in flight at each LDS
+// wait. Only for pointers into device (global) memory, never LDS.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* global_ptr(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+
+template <typename T, int CPL>
+__device__ __forceinline__ void load_raw_global(const T* p, T (&v)[CPL]) {
+  if constexpr (CPL == 1) {
+    v[0] = *global_ptr(p);
+  } else {
+    typedef typename VecT<T, CPL>::type V;
+    const V x = *reinterpret_cast<const __attribute__((address_space(1))) V*>(global_ptr(p));
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) v[c] = x[c];
+  }
+}
+
 // FP64 values: plain FP64 sum.
 struct SumF64 {
   double s;
@@ -144,13 +166,13 @@ __device__ __forceinline__ void gather_entries(const int32_t* __restrict__ idx, 
 #pragma unroll
     for (int t = 0; t < U; ++t) {
       const int32_t jt = j + t * stride;
-      ii[t] = jt < end ? idx[jt] : -1;
+      ii[t] = jt < end ? global_ptr(idx)[jt] : -1;
     }
     T v[U][CPL];
 #pragma unroll
     for (int t = 0; t < U; ++t) {
       if (ii[t] >= 0) {
-        load_raw<T, CPL>(src + (int64_t)ii[t] * ld, v[t]);
+        load_raw_global<T, CPL>(src + (int64_t)ii[t] * ld, v[t]);
       } else {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) v[t][c] = T(0);
@@ -181,7 +203,7 @@ __device__ __forceinline__ void gather_rows(const int32_t* __restrict__ idx, con
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int32_t jt = j[t] + u * stride;
-        ii[t][u] = jt < end[t] ? idx[jt] : -1;
+        ii[t][u] = jt < end[t] ? global_ptr(idx)[jt] : -1;
       }
     T v[R][U][CPL];
 #pragma unroll
@@ -189,7 +211,7 @@ __device__ __forceinline__ void gather_rows(const int32_t* __restrict__ idx, con
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (ii[t][u] >= 0) {
-          load_raw<T, CPL>(src + (int64_t)ii[t][u] * ld, v[t][u]);
+          load_raw_global<T, CPL>(src + (int64_t)ii[t][u] * ld, v[t][u]);
         } else {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) v[t][u][c] = T(0);
@@ -223,7 +245,7 @@ __device__ __forceinline__ void gather_rows_keyed(const uint32_t* __restrict__ e
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int32_t jt = j[t] + u * stride;
-        kk[t][u] = jt < end[t] ? ekey[jt] : 0xffffffffu;
+        kk[t][u] = jt < end[t] ? global_ptr(ekey)[jt] : 0xffffffffu;
       }
     T v[R][U][CPL];
 #pragma unroll
@@ -235,7 +257,7 @@ __device__ __forceinline__ void gather_rows_keyed(const uint32_t* __restrict__ e
           const int64_t r = (int64_t)(kk[t][u] & 0x3fffffffu);
           const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
           const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
-          load_raw<T, CPL>(base + r * ld, v[t][u]);
+          load_raw_global<T, CPL>(base + r * ld, v[t][u]);
         } else {
 #pragma unroll
           for (int c = 0; c < CPL; ++c) v[t][u][c] = T(0);
@@ -272,7 +294,7 @@ __device__ __forceinline__ void group_entries(const uint32_t* __restrict__ ent, 
 #pragma unroll
   for (int q = 0; q < U; ++q) {
     const int32_t i = b + q * G + lg;
-    ks[q] = (q < nslot && i < end) ? ent[i] : 0xffffffffu;
+    ks[q] = (q < nslot && i < end) ? global_ptr(ent)[i] : 0xffffffffu;
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -302,7 +324,7 @@ __device__ __forceinline__ void gather_row_keyed_group(const uint32_t* __restric
         const int64_t r = (int64_t)(kk[u] & 0x3fffffffu);
         const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
         const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
-        load_raw<T, CPL>(base + r * ld, v[u]);
+        load_raw_global<T, CPL>(base + r * ld, v[u]);
       } else {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) v[u][c] = T(0);
@@ -326,7 +348,7 @@ __device__ __forceinline__ void gather_row_indexed_group(const int32_t* __restri
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (col_ok && kk[u] != 0xffffffffu) {
-        load_raw<T, CPL>(src + (int64_t)kk[u] * ld, v[u]);
+        load_raw_global<T, CPL>(src + (int64_t)kk[u] * ld, v[u]);
       } else {
 #pragma unroll
         for (int c = 0; c < CPL; ++c) v[u][c] = T(0);
@@ -347,7 +369,7 @@ __device__ __forceinline__ void add_row_terms(const GrgHipGatherArgs& a, int64_t
     if (row >= mp.row_lo && row < mp.row_hi) {
       const int64_t r = mp.map ? (int64_t)mp.map[row - mp.row_lo] : (row - mp.row_lo);
       T v[CPL];
-      load_raw<T, CPL>(static_cast<const T*>(mp.src) + r * mp.ld + col, v);
+      load_raw_global<T, CPL>(static_cast<const T*>(mp.src) + r * mp.ld + col, v);
 #pragma unroll
       for (int c = 0; c < CPL; ++c) acc[c].add(v[c]);
     }
@@ -362,7 +384,7 @@ __device__ __forceinline__ void add_row_terms(const GrgHipGatherArgs& a, int64_t
   }
   if (a.accumulate) {
     T v[CPL];
-    load_raw<T, CPL>(static_cast<const T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
+    load_raw_global<T, CPL>(static_cast<const T*>(a.out) + (row - a.out_base) * a.ld_out + col, v);
 #pragma unroll
     for (int c = 0; c < CPL; ++c) acc[c].add(v[c]);
   }
@@ -684,7 +706,7 @@ __device__ __forceinline__ void stream_gather(const uint32_t (&kk)[U], const T* 
       const int64_t r = (int64_t)(kk[u] & 0x3fffffffu);
       const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
       const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
-      load_raw<T, CPL>(base + r * ld, v[u]);
+      load_raw_global<T, CPL>(base + r * ld, v[u]);
     } else {
 #pragma unroll
       for (int c = 0; c < CPL; ++c) v[u][c] = T(0);
@@ -908,7 +930,7 @@ __global__ __launch_bounds__(kBlock) void tile_gather_kernel(const GrgHipGatherA
           const int64_t srow = (int64_t)(key & 0x3fffffffu);
           const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
           const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
-          load_raw<T, CPL>(base + srow * ld + col, v[u]);
+          load_raw_global<T, CPL>(base + srow * ld + col, v[u]);
           dst[u] = d * stride + p * CPL;
         }
       }
@@ -979,50 +1001,54 @@ __global__ __launch_bounds__(kBlock) void tile_gather_kernel(const GrgHipGatherA
 
 // ---- panel launch -----------------------------------------------------------
 
-// LDS of a panel workgroup, in bytes: the panel's pass entry offsets, one
-// pass's staged rows, its row offsets (uint16) and its entry positions (uint8,
-// plus 32 bytes of slack for the 16-byte word alignment of the copy).
+constexpr int kPanelWords = 1;  // 16-byte words of one pass's entry positions per thread (4 KiB per pass)
+
+// LDS of a panel workgroup, in bytes: the panel's pass source and entry offsets
+// (int32 each), then two pass buffers, each one pass's staged rows, its row
+// offsets (uint16) and its entry positions (uint8, kPanelWords words per thread).
 struct PanelLds {
   int64_t meta, staged, rowoff, entries;
-  __host__ __device__ int64_t total() const { return meta + staged + rowoff + entries; }
+  __host__ __device__ int64_t buffer() const { return staged + rowoff + entries; }
+  __host__ __device__ int64_t total() const { return meta + 2 * buffer(); }
 };
 
 __host__ __device__ inline PanelLds panel_lds_layout(int64_t max_passes, int64_t staged_rows, int64_t stride_bytes,
-                                                     int64_t panel_rows, int64_t max_entries) {
+                                                     int64_t panel_rows) {
   PanelLds t;
-  t.meta = round16((max_passes + 1) * 4);
+  t.meta = round16(2 * (max_passes + 1) * 4);
   t.staged = round16(staged_rows * stride_bytes);
   t.rowoff = round16((panel_rows + 1) * 2);
-  t.entries = round16(max_entries + 32);
+  t.entries = (int64_t)kPanelWords * 16 * kBlock;
   return t;
 }
 
-constexpr int kPanelWords = 4;  // 16-byte words of entry positions per thread and pass, at most
+typedef uint32_t PanelWord __attribute__((ext_vector_type(4)));
 
-// Staged pieces (kl-lane vectors) of pass q of a panel whose sources are
-// pan_src[sbase .. send): every pass but the last stages panel_staged rows.
-__device__ __forceinline__ int panel_pieces(const GrgHipGatherArgs& a, int32_t sbase, int32_t send, int q, int kl_shift) {
-  const int32_t s0 = sbase + q * a.panel_staged;
-  const int32_t n = send - s0;
-  return (n < a.panel_staged ? n : a.panel_staged) << kl_shift;
-}
-
+// Keys of pass j's staged pieces (kl-lane vectors): the pass stages sources
+// pan_src[s0 .. s0 + n) (n >= 1), at most panel_staged of them. Every thread
+// loads a key for each of its PPT pieces (absent pieces reload the pass's last
+// key and are marked 0xffffffff), so the load count never depends on the pass.
 template <int PPT>
-__device__ __forceinline__ void panel_keys(const GrgHipGatherArgs& a, int32_t sbase, int32_t send, int q, int kl_shift,
-                                           uint32_t (&key)[PPT]) {
-  const int32_t s0 = sbase + q * a.panel_staged;
-  const int n_pieces = panel_pieces(a, sbase, send, q, kl_shift);
+__device__ __forceinline__ void panel_keys(const GrgHipGatherArgs& a, int32_t s0, int32_t n, int kl_shift, uint32_t (&key)[PPT]) {
+  const int n_pieces = (n < a.panel_staged ? n : a.panel_staged) << kl_shift;
+  const auto src = global_ptr(a.pan_src);
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
     const int i = threadIdx.x + u * kBlock;
-    key[u] = i < n_pieces ? a.pan_src[s0 + (i >> kl_shift)] : 0xffffffffu;
+    const int r = i < n_pieces ? (i >> kl_shift) : (n > 0 ? n - 1 : 0);
+    const uint32_t k = src[s0 + r];
+    key[u] = i < n_pieces ? k : 0xffffffffu;
   }
 }
 
-// Source row pieces of one pass, by key, into registers.
+// Loads of one pass into registers: the staged pieces by key (absent pieces and
+// columns past k load row 0, column 0 of the first segment and are zeroed), the
+// pass's entry positions [e0, e1) as 16-byte words, and one row offset. Branch
+// free, so the compiler waits for the oldest pass in flight alone.
 template <typename T, int CPL, int PPT>
-__device__ __forceinline__ void panel_rows_load(const GrgHipGatherArgs& a, const uint32_t (&key)[PPT], int kl_shift, int col0,
-                                                T (&v)[PPT][CPL]) {
+__device__ __forceinline__ void panel_issue(const GrgHipGatherArgs& a, const uint32_t (&key)[PPT], int col0, int kl_shift,
+                                            int32_t e0, int32_t e1, int64_t rowoff_at, T (&v)[PPT][CPL],
+                                            PanelWord (&w)[kPanelWords], uint32_t& roff) {
   const int KL = 1 << kl_shift;
   const T* src0 = static_cast<const T*>(a.seg[0].src);
   const T* src1 = a.nseg > 1 ? static_cast<const T*>(a.seg[1].src) : src0;
@@ -1033,42 +1059,39 @@ __device__ __forceinline__ void panel_rows_load(const GrgHipGatherArgs& a, const
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
     const int i = threadIdx.x + u * kBlock;
-    const int col = col0 + (i & (KL - 1)) * CPL;
-    if (key[u] != 0xffffffffu && col < a.k) {
-      const uint32_t sg = key[u] >> 30;
-      const int64_t srow = (int64_t)(key[u] & 0x3fffffffu);
-      const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
-      const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
-      load_raw<T, CPL>(base + srow * ld + col, v[u]);
-    } else {
+    const int c = col0 + (i & (KL - 1)) * CPL;
+    const bool ok = key[u] != 0xffffffffu && c < a.k;
+    const uint32_t kk = ok ? key[u] : 0u;
+    const uint32_t sg = kk >> 30;
+    const int64_t srow = (int64_t)(kk & 0x3fffffffu);
+    const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
+    const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
+    T tmp[CPL];
+    load_raw_global<T, CPL>(base + srow * ld + (ok ? c : 0), tmp);
 #pragma unroll
-      for (int c = 0; c < CPL; ++c) v[u][c] = T(0);
-    }
+    for (int cc = 0; cc < CPL; ++cc) v[u][cc] = ok ? tmp[cc] : T(0);
   }
-}
-
-typedef uint32_t PanelWord __attribute__((ext_vector_type(4)));
-
-// Pass g's row offsets and its entry positions [e0, e1), as 16-byte words, into registers.
-__device__ __forceinline__ void panel_record_load(const GrgHipGatherArgs& a, int64_t g, int32_t e0, int32_t e1, int ent_words,
-                                                  PanelWord (&w)[kPanelWords], uint16_t& roff) {
-  const int R = a.panel_rows;
   const int32_t w0 = e0 >> 4;
   int32_t nw = ((e1 + 15) >> 4) - w0;
-  if (nw > ent_words) nw = ent_words;  // never more than the LDS holds
-  const PanelWord* src = reinterpret_cast<const PanelWord*>(a.pan_lidx) + w0;
+  if (nw > kPanelWords * kBlock) nw = kPanelWords * kBlock;  // never more than the LDS holds
+  const auto ent = global_ptr(reinterpret_cast<const PanelWord*>(a.pan_lidx));
 #pragma unroll
   for (int u = 0; u < kPanelWords; ++u) {
     const int i = threadIdx.x + u * kBlock;
-    w[u] = i < nw ? src[i] : PanelWord{0u, 0u, 0u, 0u};
+    const PanelWord word = ent[w0 + (i < nw ? i : 0)];  // word w0 is readable (16 bytes of padding)
+    w[u] = i < nw ? word : PanelWord{0u, 0u, 0u, 0u};
   }
-  roff = threadIdx.x <= R ? a.pass_rowoff[g * (R + 1) + threadIdx.x] : (uint16_t)0;
+  const int R = a.panel_rows;
+  const int tr = (int)threadIdx.x <= R ? (int)threadIdx.x : R;
+  const uint32_t ro = (uint32_t)global_ptr(a.pass_rowoff)[rowoff_at + tr];
+  roff = (int)threadIdx.x <= R ? ro : 0u;
 }
 
+// One pass from registers into an LDS pass buffer.
 template <typename T, int CPL, int PPT>
-__device__ __forceinline__ void panel_store_pass(const GrgHipGatherArgs& a, int n_pieces, int kl_shift, int stride,
-                                                 const T (&v)[PPT][CPL], const PanelWord (&w)[kPanelWords], uint16_t roff,
-                                                 int ent_words, T* buf, uint16_t* rowoff, PanelWord* ent) {
+__device__ __forceinline__ void panel_store(int n_pieces, int kl_shift, int stride, int R, const T (&v)[PPT][CPL],
+                                            const PanelWord (&w)[kPanelWords], uint32_t roff, T* buf, uint16_t* rowoff,
+                                            PanelWord* ent) {
   const int KL = 1 << kl_shift;
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
@@ -1076,21 +1099,55 @@ __device__ __forceinline__ void panel_store_pass(const GrgHipGatherArgs& a, int 
     if (i < n_pieces) store_raw<T, CPL>(buf + (i >> kl_shift) * stride + (i & (KL - 1)) * CPL, v[u]);
   }
 #pragma unroll
-  for (int u = 0; u < kPanelWords; ++u) {
-    const int i = threadIdx.x + u * kBlock;
-    if (i < ent_words) ent[i] = w[u];  // the entry region holds ent_words words
+  for (int u = 0; u < kPanelWords; ++u) ent[threadIdx.x + u * kBlock] = w[u];
+  if ((int)threadIdx.x <= R) rowoff[threadIdx.x] = (uint16_t)roff;
+}
+
+// A lane group's row: its entries of one stored pass, summed from LDS.
+template <typename T, int CPL, typename Acc>
+__device__ __forceinline__ void panel_sum(const T* lds_col, const uint16_t* rowoff, const uint8_t* ent, int gid, int e, int ES,
+                                          int stride, Acc (&acc)[CPL]) {
+  const int p0 = rowoff[gid], p1 = rowoff[gid + 1];
+  for (int jj = p0 + e; jj < p1; jj += 4 * ES) {
+    int li[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = jj + u * ES;
+      li[u] = k < p1 ? (int)ent[k] : -1;
+    }
+    T x[4][CPL];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (li[u] >= 0) {
+        load_raw<T, CPL>(lds_col + li[u] * stride, x[u]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) x[u][c] = T(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < CPL; ++c) acc[c].add(x[u][c]);
   }
-  if (threadIdx.x <= a.panel_rows) rowoff[threadIdx.x] = roff;
 }
 
 // Panel launch: workgroup t owns panel t (panel_rows consecutive rows, one per
-// lane group). The panel's pass entry offsets are read into LDS once; each
-// pass then stages panel_staged of the panel's distinct source rows in LDS
-// with the pass's entry lists, while the next pass's source rows, entry lists
-// and the following pass's keys are in flight. Each lane group keeps its row's
-// sums in registers over all passes and writes the row once. Heavy rows follow
-// as ordinary workgroup rows.
-template <typename T, int CPL, int PPT>
+// lane group; t in the launch's XCD block order). The panel's pass offsets are
+// read into LDS once. Its passes then stream through a pipeline: D passes are
+// in flight in registers (each thread's pieces of the pass's staged source rows,
+// its words of the pass's entry positions and one row offset) while the pass
+// before them is summed from one of two LDS buffers; the keys of the next pass
+// to issue are loaded one pass ahead. Per pass: store the oldest pass in flight
+// to the buffer the pass before last used, one barrier, issue the pass D ahead
+// into the freed registers (past the panel's last pass, the last pass again:
+// the load count stays the same and those loads are never stored), sum the
+// stored pass. The barrier also orders the buffer's reuse: every thread
+// finished summing from it (two passes earlier) before any thread reaches the
+// barrier of the pass in between. Each lane group keeps its row's sums in
+// registers over all passes and writes the row once. Heavy rows follow as
+// ordinary workgroup rows.
+template <typename T, int CPL, int PPT, int D>
 __global__ __launch_bounds__(kBlock) void panel_gather_kernel(const GrgHipGatherArgs a, int g_shift, int kl_shift) {
   using Acc = typename SumOf<T>::type;
   extern __shared__ __align__(16) unsigned char panel_lds[];
@@ -1103,22 +1160,16 @@ __global__ __launch_bounds__(kBlock) void panel_gather_kernel(const GrgHipGather
     workgroup_row<T, CPL, 8>(a, a.heavy_rows[t - np], t - np, (int)blockIdx.y, kl_shift, partial);
     return;
   }
-  if (a.flags & GRG_HIP_FLAG_XCD_REMAP) {
-    const int64_t q = np >> 3, r = np & 7, x = t & 7;
-    t = x * q + (x < r ? x : r) + (t >> 3);
-  }
+  t = xcd_order(t, np, a.flags);
   const int R = a.panel_rows;
   const int chunk = KL * CPL;
   const int stride = chunk + tile_pad<T, CPL>();
   const int col0 = (int)blockIdx.y * chunk;
-  const PanelLds lay = panel_lds_layout(a.panel_max_passes, a.panel_staged, (int64_t)stride * (int64_t)sizeof(T), R,
-                                        a.panel_max_entries);
-  int32_t* meta = reinterpret_cast<int32_t*>(panel_lds);
-  T* buf = reinterpret_cast<T*>(panel_lds + lay.meta);
-  uint16_t* rowoff = reinterpret_cast<uint16_t*>(panel_lds + lay.meta + lay.staged);
-  PanelWord* ent_w = reinterpret_cast<PanelWord*>(panel_lds + lay.meta + lay.staged + lay.rowoff);
-  const uint8_t* ent = reinterpret_cast<const uint8_t*>(ent_w);
-  const int ent_words = (int)(lay.entries >> 4);
+  const PanelLds lay = panel_lds_layout(a.panel_max_passes, a.panel_staged, (int64_t)stride * (int64_t)sizeof(T), R);
+  int32_t* msrc = reinterpret_cast<int32_t*>(panel_lds);
+  int32_t* ment = msrc + (a.panel_max_passes + 1);
+  unsigned char* buf0 = panel_lds + lay.meta;
+  unsigned char* buf1 = buf0 + lay.buffer();
 
   const int G = 1 << g_shift;
   const int lg = lane & (G - 1);
@@ -1137,70 +1188,83 @@ __global__ __launch_bounds__(kBlock) void panel_gather_kernel(const GrgHipGather
     }
     live = deg <= a.heavy_threshold;  // a heavy row is produced by its heavy workgroup
   }
-  const T* lds_col = buf + cl * CPL;
   Acc acc[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) acc[c].zero();
 
-  // the panel's passes, their sources pan_src[sbase .. send) and entry offsets (in LDS)
   const int32_t g0 = a.pan_pass[t];
   int npass = a.pan_pass[t + 1] - g0;
   if (npass > a.panel_max_passes) npass = a.panel_max_passes;  // never more than the LDS holds
-  const int32_t sbase = a.pass_src[g0], send = a.pass_src[g0 + npass];
-  for (int i = threadIdx.x; i <= npass; i += kBlock) meta[i] = a.pass_ent[g0 + i];
-  __syncthreads();
-
-  uint32_t key[PPT];
-  T v[PPT][CPL];
-  PanelWord w[kPanelWords];
-  uint16_t roff = 0;
-  if (npass > 0) {
-    panel_keys<PPT>(a, sbase, send, 0, kl_shift, key);
-    panel_record_load(a, g0, meta[0], meta[1], ent_words, w, roff);
-    panel_rows_load<T, CPL, PPT>(a, key, kl_shift, col0, v);
-    panel_store_pass<T, CPL, PPT>(a, panel_pieces(a, sbase, send, 0, kl_shift), kl_shift, stride, v, w, roff, ent_words, buf,
-                                  rowoff, ent_w);
-    if (npass > 1) panel_keys<PPT>(a, sbase, send, 1, kl_shift, key);
+  for (int i = threadIdx.x; i <= npass; i += kBlock) {
+    msrc[i] = a.pass_src[g0 + i];
+    ment[i] = a.pass_ent[g0 + i];
   }
   __syncthreads();
-  for (int q = 0; q < npass; ++q) {
-    // loads of pass q + 1 (and the keys of pass q + 2) fly while pass q is summed
-    if (q + 1 < npass) {
-      panel_rows_load<T, CPL, PPT>(a, key, kl_shift, col0, v);
-      panel_record_load(a, g0 + q + 1, meta[q + 1], meta[q + 2], ent_words, w, roff);
-      if (q + 2 < npass) panel_keys<PPT>(a, sbase, send, q + 2, kl_shift, key);
+
+  if (npass > 0) {  // the same for every thread of the workgroup
+    // pass p's registers: v / w / roff[p % D]; its keys: key[p % D], loaded before the
+    // rows of the pass before it, so waiting for the keys leaves those rows in flight
+    T v[D][PPT][CPL];
+    PanelWord w[D][kPanelWords];
+    uint32_t roff[D];
+    uint32_t key[D][PPT];
+    const int last = npass - 1;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int p = j < last ? j : last;
+      panel_keys<PPT>(a, msrc[p], msrc[p + 1] - msrc[p], kl_shift, key[j]);
     }
-    if (live) {
-      const int p0 = rowoff[gid], p1 = rowoff[gid + 1];
-      const uint8_t* ent_q = ent + (meta[q] & 15);
-      for (int j = p0 + e; j < p1; j += 4 * ES) {
-        int li[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int jj = j + u * ES;
-          li[u] = jj < p1 ? (int)ent_q[jj] : -1;
-        }
-        T x[4][CPL];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (li[u] >= 0) {
-            load_raw<T, CPL>(lds_col + li[u] * stride, x[u]);
-          } else {
-#pragma unroll
-            for (int c = 0; c < CPL; ++c) x[u][c] = T(0);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int c = 0; c < CPL; ++c) acc[c].add(x[u][c]);
+    for (int j = 0; j < D; ++j) {
+      const int p = j < last ? j : last;
+      panel_issue<T, CPL, PPT>(a, key[j], col0, kl_shift, ment[p], ment[p + 1], (int64_t)(g0 + p) * (R + 1), v[j], w[j], roff[j]);
+      if (j == 0) {
+        const int pk = D < last ? D : last;  // pass D's keys take pass 0's key slot
+        panel_keys<PPT>(a, msrc[pk], msrc[pk + 1] - msrc[pk], kl_shift, key[0]);
       }
     }
-    __syncthreads();
-    if (q + 1 < npass)
-      panel_store_pass<T, CPL, PPT>(a, panel_pieces(a, sbase, send, q + 1, kl_shift), kl_shift, stride, v, w, roff, ent_words,
-                                    buf, rowoff, ent_w);
-    __syncthreads();
+    const T* lds_col0 = reinterpret_cast<const T*>(buf0) + cl * CPL;
+    const T* lds_col1 = reinterpret_cast<const T*>(buf1) + cl * CPL;
+    int q0 = 0;
+    for (; q0 + D <= npass; q0 += D) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const int q = q0 + j;
+        unsigned char* b = (q & 1) ? buf1 : buf0;
+        const int n_q = msrc[q + 1] - msrc[q];
+        panel_store<T, CPL, PPT>((n_q < a.panel_staged ? n_q : a.panel_staged) << kl_shift, kl_shift, stride, R, v[j], w[j],
+                                 roff[j], reinterpret_cast<T*>(b), reinterpret_cast<uint16_t*>(b + lay.staged),
+                                 reinterpret_cast<PanelWord*>(b + lay.staged + lay.rowoff));
+        __syncthreads();
+        // keys of pass q + D + 1 (slot (j + 1) % D, whose pass q + 1 was issued already), then
+        // the rows of pass q + D (slot j)
+        const int pk = q + D + 1 < last ? q + D + 1 : last;
+        panel_keys<PPT>(a, msrc[pk], msrc[pk + 1] - msrc[pk], kl_shift, key[(j + 1) % D]);
+        const int p = q + D < last ? q + D : last;
+        panel_issue<T, CPL, PPT>(a, key[j], col0, kl_shift, ment[p], ment[p + 1], (int64_t)(g0 + p) * (R + 1), v[j], w[j],
+                                 roff[j]);
+        if (live)
+          panel_sum<T, CPL, Acc>((q & 1) ? lds_col1 : lds_col0, reinterpret_cast<const uint16_t*>(b + lay.staged),
+                                 reinterpret_cast<const uint8_t*>(b + lay.staged + lay.rowoff) + (ment[q] & 15), gid, e, ES,
+                                 stride, acc);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int q = q0 + j;
+      if (q < npass) {
+        unsigned char* b = (q & 1) ? buf1 : buf0;
+        const int n_q = msrc[q + 1] - msrc[q];
+        panel_store<T, CPL, PPT>((n_q < a.panel_staged ? n_q : a.panel_staged) << kl_shift, kl_shift, stride, R, v[j], w[j],
+                                 roff[j], reinterpret_cast<T*>(b), reinterpret_cast<uint16_t*>(b + lay.staged),
+                                 reinterpret_cast<PanelWord*>(b + lay.staged + lay.rowoff));
+        __syncthreads();
+        if (live)
+          panel_sum<T, CPL, Acc>((q & 1) ? lds_col1 : lds_col0, reinterpret_cast<const uint16_t*>(b + lay.staged),
+                                 reinterpret_cast<const uint8_t*>(b + lay.staged + lay.rowoff) + (ment[q] & 15), gid, e, ES,
+                                 stride, acc);
+      }
+    }
   }
   for (int off = KL; off < G; off <<= 1) {
 #pragma unroll
@@ -1209,6 +1273,152 @@ __global__ __launch_bounds__(kBlock) void panel_gather_kernel(const GrgHipGather
   if (live && e == 0) {
     add_row_terms<T, CPL>(a, row, col, acc);
     store_row<T, CPL>(a, row, col, acc);
+  }
+}
+
+
+// ---- row-group launch (FP64, MFMA) ------------------------------------------
+
+// The 16 x 16 FP64 tile of one MFMA (v_mfma_f64_16x16x4f64): lane j + 16 (i % 4),
+// register i / 4 holds tile[i][j]. Its operands: lane i + 16 q holds A[i][q]
+// and B[q][i] (as measured on gfx950 with integer-valued products).
+typedef double GroupTile __attribute__((ext_vector_type(4)));
+
+// Rounds r = r0, r0 + 4, r0 + 8, ... < q1 of four group entries each, summed into
+// one wave's 16 x 16 tile (rows of the group x the 16 columns of the chunk whose
+// column col this lane holds). Per round, lane i + 16 q holds entry q's
+// membership bit of row i (A, 16 x 4) and column col of entry q's source row
+// (B, 4 x 16); one MFMA adds A x B, i.e. every source row read once is added to
+// each row of its mask. The rounds go in batches of U: a batch's keys and masks
+// (4 U entries) are loaded by the whole wave, one or two per lane, into the
+// wave's LDS slice (keys[q * U + u], masks likewise), while the previous batch's
+// rows are still in flight; each lane then reads its U keys back, gathers the U
+// rows, and reads each mask just before its MFMA. Registers thus hold little
+// besides the gathered values (U doubles per lane). Two tiles take alternate
+// rounds so that consecutive MFMAs do not wait on one another.
+template <int U>
+__device__ __forceinline__ GroupTile group_tile(const GrgHipGatherArgs& a, int64_t r0, int64_t q1, int col, bool col_ok, int i,
+                                                int q, int lane, uint32_t* skey, uint16_t* smask) {
+  static_assert(U % 16 == 0, "a batch is a multiple of 16 rounds (64 entries, one per lane)");
+  constexpr int kPer = 4 * U / 64;  // staged entries per lane and batch
+  const double* src0 = static_cast<const double*>(a.seg[0].src);
+  const double* src1 = a.nseg > 1 ? static_cast<const double*>(a.seg[1].src) : src0;
+  const double* src2 = a.nseg > 2 ? static_cast<const double*>(a.seg[2].src) : src0;
+  const int64_t ld0 = a.seg[0].ld;
+  const int64_t ld1 = a.nseg > 1 ? a.seg[1].ld : ld0;
+  const int64_t ld2 = a.nseg > 2 ? a.seg[2].ld : ld0;
+  GroupTile t0 = {0.0, 0.0, 0.0, 0.0};
+  GroupTile t1 = {0.0, 0.0, 0.0, 0.0};
+  // entry slot t = lane + 64 p of a batch: round u = t >> 2, entry qq = t & 3
+  uint32_t nk[kPer];
+  uint32_t nm[kPer];
+  auto fetch = [&](int64_t r) {
+#pragma unroll
+    for (int p = 0; p < kPer; ++p) {
+      const int t = lane + 64 * p;
+      const int64_t rr = r + 4 * (int64_t)(t >> 2);
+      if (rr < q1) {
+        nk[p] = a.grp_key[rr * 4 + (t & 3)];
+        nm[p] = (uint32_t)a.grp_mask[rr * 4 + (t & 3)];
+      } else {
+        nk[p] = 0xffffffffu;
+        nm[p] = 0u;
+      }
+    }
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int p = 0; p < kPer; ++p) {
+      const int t = lane + 64 * p;
+      skey[(t & 3) * U + (t >> 2)] = nk[p];
+      smask[(t & 3) * U + (t >> 2)] = (uint16_t)nm[p];
+    }
+    // the slice is this wave's alone: order its writes before the lanes' reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  fetch(r0);
+  for (int64_t r = r0; r < q1; r += 4 * U) {
+    stage();
+    double b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t key = skey[q * U + u];
+      const uint32_t sg = key >> 30;
+      b[u] = 0.0;
+      if (sg != 3u && col_ok) {
+        const int64_t srow = (int64_t)(key & 0x3fffffffu);
+        const double* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
+        const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
+        b[u] = base[srow * ld + col];
+      }
+    }
+    fetch(r + 4 * U);  // the next batch's keys fly while these rows arrive
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double av = ((smask[q * U + u] >> i) & 1u) ? 1.0 : 0.0;
+      if (u & 1) t1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[u], t1, 0, 0, 0);
+      else t0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[u], t0, 0, 0, 0);
+    }
+    // every lane has read this batch's keys and masks before the next stage() overwrites them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  return t0 + t1;
+}
+
+// Row-group launch: workgroup b sums group b' (b in the launch's XCD block
+// order) per 16-column chunk. Its four waves take interleaved rounds of the
+// group's entries (wave w: rounds w, w + 4, ...), each into a tile of its own;
+// waves 1-3 hand their tiles to wave 0 through LDS, wave 0 adds them in wave
+// order and stores the rows of the group's store mask with their row terms.
+// Workgroups past the groups' produce the heavy rows. Every call of the MFMA is
+// made by all 64 lanes of a wave: the round loop's bounds are the wave's.
+template <int CPL, int U>
+__global__ __launch_bounds__(kBlock) void group_gather_kernel(const GrgHipGatherArgs a, int kl_shift) {
+  __shared__ SumF64 partial[kWavesPerBlock][64];
+  __shared__ double handoff[kWavesPerBlock - 1][4][64];
+  __shared__ uint32_t skey[kWavesPerBlock][4 * U];
+  __shared__ uint16_t smask[kWavesPerBlock][4 * U];
+  const int64_t bx = blockIdx.x;
+  const int64_t ng = a.num_groups;
+  if (bx >= ng) {
+    const int64_t h = bx - ng;
+    workgroup_row<double, CPL, 8>(a, a.heavy_rows[h], h, (int)blockIdx.y, kl_shift, partial);
+    return;
+  }
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int64_t g = xcd_order(bx, ng, a.flags);
+  const int i = lane & 15;
+  const int q = lane >> 4;
+  const int col = (int)blockIdx.y * 16 + i;
+  const bool col_ok = col < a.k;
+  GroupTile t = group_tile<U>(a, (a.grp_off[g] >> 2) + w, a.grp_off[g + 1] >> 2, col, col_ok, i, q, lane, skey[w], smask[w]);
+  if (w > 0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) handoff[w - 1][r][lane] = t[r];
+  }
+  __syncthreads();
+  if (w != 0) return;
+#pragma unroll
+  for (int v = 0; v < kWavesPerBlock - 1; ++v) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) t[r] += handoff[v][r][lane];
+  }
+  const uint32_t store = a.grp_store[g];
+  const int64_t row0 = a.row_begin + g * 16;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int il = q + 4 * r;  // the tile row in register r of this lane
+    if (col_ok && ((store >> il) & 1u)) {
+      SumF64 acc[1];
+      acc[0].s = t[r];
+      add_row_terms<double, 1>(a, row0 + il, col, acc);
+      store_row<double, 1>(a, row0 + il, col, acc);
+    }
   }
 }
 
@@ -1371,7 +1581,7 @@ int64_t panel_lds_bytes(const GrgHipGatherArgs& a) {
   const int64_t pad = a.cpl * elem >= 16 ? a.cpl : 16 / elem;
   const int64_t stride = (int64_t)a.kl * a.cpl + pad;
   const int64_t stat = f64 ? tile_static_lds<double>() : tile_static_lds<float>();
-  return panel_lds_layout(a.panel_max_passes, a.panel_staged, stride * elem, a.panel_rows, a.panel_max_entries).total() + stat;
+  return panel_lds_layout(a.panel_max_passes, a.panel_staged, stride * elem, a.panel_rows).total() + stat;
 }
 
 int validate(const GrgHipGatherArgs& a) {
@@ -1390,8 +1600,21 @@ int validate(const GrgHipGatherArgs& a) {
   }
   if (a.flags & ~(GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_MERGED |
                   GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_XCD_RUNS | (15 << GRG_HIP_XCD_RUN_SHIFT) |
-                  GRG_HIP_FLAG_GROUP_KEYS))
+                  GRG_HIP_FLAG_GROUP_KEYS | GRG_HIP_FLAG_GROUPS))
     return fail("unknown flags");
+  if (a.flags & GRG_HIP_FLAG_GROUPS) {
+    if (a.dtype != GRG_HIP_F64) return fail("the row-group launch sums FP64 values");
+    if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_MERGED | GRG_HIP_FLAG_GROUP_KEYS |
+                   GRG_HIP_FLAG_PARTIALS))
+      return fail("a row-group launch is not tiled, panel, streamed, merged, group-keyed or a partials launch");
+    if (a.num_groups < 0) return fail("num_groups < 0");
+    if (a.num_groups > 0 && (!a.grp_off || !a.grp_store || !a.grp_key || !a.grp_mask))
+      return fail("row-group launch needs grp_off, grp_store, grp_key and grp_mask");
+    if (a.num_groups * 16 < a.row_end - a.row_begin) return fail("row-group launch: the groups do not cover the rows");
+    if (a.kl * a.cpl != 16) return fail("row-group launch: kl * cpl must be 16 (one 16-column chunk)");
+    if (a.nseg < 1) return fail("row-group launch needs at least one segment");
+    if (a.unroll != 0 && a.unroll != 4 && a.unroll != 8 && a.unroll != 16) return fail("row-group launch: unroll must be 4, 8 or 16");
+  }
   if (a.flags & GRG_HIP_FLAG_GROUP_KEYS) {
     if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_PARTIALS))
       return fail("group-loaded entries apply to the per-row light path (not tiled, panel, streamed or partials)");
@@ -1499,19 +1722,25 @@ int launch_tiled_typed(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl
   return 0;
 }
 
-template <typename T, int CPL, int PPT>
-int launch_panel_pieces(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl_shift, hipStream_t s) {
+template <typename T, int CPL, int PPT, int D>
+int launch_panel_depth(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl_shift, hipStream_t s) {
   const size_t dyn = (size_t)(panel_lds_bytes(a) - tile_static_lds<T>());
   if (dyn > 64 * 1024) {
-    hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_gather_kernel<T, CPL, PPT>),
+    hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_gather_kernel<T, CPL, PPT, D>),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     if (err != hipSuccess) return fail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(err), 2);
   }
-  hipLaunchKernelGGL((panel_gather_kernel<T, CPL, PPT>), grid, dim3(kBlock), dyn, s, a, g_shift, kl_shift);
+  hipLaunchKernelGGL((panel_gather_kernel<T, CPL, PPT, D>), grid, dim3(kBlock), dyn, s, a, g_shift, kl_shift);
   return 0;
 }
 
-// pieces staged per thread and pass: panel_staged * kl / 256, rounded up to 1, 2 or 4
+// passes in flight: 2 for unroll 4, else 3
+template <typename T, int CPL, int PPT>
+int launch_panel_pieces(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl_shift, hipStream_t s) {
+  if (a.unroll == 4) return launch_panel_depth<T, CPL, PPT, 2>(a, grid, g_shift, kl_shift, s);
+  return launch_panel_depth<T, CPL, PPT, 3>(a, grid, g_shift, kl_shift, s);
+}
+
 template <typename T, int CPL>
 int launch_panel_typed(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl_shift, hipStream_t s) {
   const int64_t pieces = ((int64_t)a.panel_staged * a.kl + kBlock - 1) / kBlock;
@@ -1541,11 +1770,34 @@ int launch_stream_typed(const GrgHipGatherArgs& a, int64_t want_blocks, int gy, 
   return 0;
 }
 
+template <int CPL>
+void launch_groups_typed(const GrgHipGatherArgs& a, dim3 grid, int kl_shift, hipStream_t s) {
+  // unroll 4 / 8 / 16 select batches of 16 / 32 / 48 rounds
+  const int u = a.unroll ? a.unroll : 8;
+  if (u == 4) hipLaunchKernelGGL((group_gather_kernel<CPL, 16>), grid, dim3(kBlock), 0, s, a, kl_shift);
+  else if (u == 16) hipLaunchKernelGGL((group_gather_kernel<CPL, 48>), grid, dim3(kBlock), 0, s, a, kl_shift);
+  else hipLaunchKernelGGL((group_gather_kernel<CPL, 32>), grid, dim3(kBlock), 0, s, a, kl_shift);
+}
+
 int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
   if (int rc = validate(a)) return rc;
   const int G = a.kl * a.es;
   const int g_shift = ilog2(G), kl_shift = ilog2(a.kl);
   int64_t n_light = 0, gx = 0;
+  if (a.flags & GRG_HIP_FLAG_GROUPS) {
+    gx = a.num_groups + a.num_heavy;
+    if (gx == 0) return 0;
+    if (gx > 0x7fffffffLL) return fail("grid too large");
+    dim3 grid((unsigned)gx, (unsigned)((a.k + 15) / 16));
+    switch (a.cpl) {
+      case 1: launch_groups_typed<1>(a, grid, kl_shift, stream); break;
+      case 2: launch_groups_typed<2>(a, grid, kl_shift, stream); break;
+      default: launch_groups_typed<4>(a, grid, kl_shift, stream); break;
+    }
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return fail(std::string("kernel launch failed: ") + hipGetErrorString(err), 2);
+    return 0;
+  }
   if (a.flags & GRG_HIP_FLAG_STREAM) {
     const int64_t rows_per_block = (int64_t)kWavesPerBlock * (64 / G);
     const int64_t want = std::max<int64_t>((a.row_end - a.row_begin + rows_per_block - 1) / rows_per_block, a.num_heavy);
@@ -1727,8 +1979,8 @@ int grg_hip_jobs_build(const GrgHipGatherArgs* args, const int32_t* phase_of, in
       g_last_error = "job " + std::to_string(i) + ": " + g_last_error;
       return rc;
     }
-    if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM))
-      return fail("job " + std::to_string(i) + ": tiled, panel and streamed launches are not run from job tables");
+    if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_GROUPS))
+      return fail("job " + std::to_string(i) + ": tiled, panel, streamed and row-group launches are not run from job tables");
     if (a.dtype != args[0].dtype) return fail("jobs of one table share one value type");
     const int32_t p = phase_of[i];
     if (p < 0 || p >= nphase || p < cur) return fail("phase_of must be non-decreasing and in [0, nphase)");

@@ -459,3 +459,145 @@ int grg_host_panels_copy(const void* panels, int32_t* pan_pass, int32_t* pass_sr
 void grg_host_panels_free(void* panels) { delete static_cast<PanelSet*>(panels); }
 
 }  // extern "C"
+
+// ---- row groups of the MFMA row-group launch ---------------------------------
+
+namespace {
+
+constexpr int kGroupRows = 16;
+constexpr uint32_t kAbsentKey = 0xffffffffu;
+
+struct GroupPart {
+  std::vector<int64_t> count;   // padded union entries per group
+  std::vector<uint16_t> store;  // rows each group produces
+  std::vector<uint32_t> key;
+  std::vector<uint16_t> mask;
+};
+
+struct GroupSet {
+  std::vector<int64_t> off{0};
+  std::vector<uint16_t> store;
+  std::vector<uint32_t> key;
+  std::vector<uint16_t> mask;
+};
+
+// Groups [g0, g1) of rows [row_begin, row_end): each group's light rows' entries
+// as (key << 16 | row bit), sorted, then runs of one key merged into one entry
+// whose mask holds the bits of the rows that list it.
+void build_group_part(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys, int64_t heavy,
+                      int64_t g0, int64_t g1, GroupPart& out) {
+  std::vector<uint64_t> items;
+  for (int64_t g = g0; g < g1; ++g) {
+    const int64_t r0 = row_begin + g * kGroupRows;
+    const int64_t r1 = std::min<int64_t>(row_end, r0 + kGroupRows);
+    items.clear();
+    uint16_t store = 0;
+    for (int64_t r = r0; r < r1; ++r) {
+      const int64_t deg = (int64_t)eptr[r + 1] - eptr[r];
+      if (deg < 0) throw std::invalid_argument("eptr is not non-decreasing");
+      if (heavy >= 0 && deg > heavy) continue;  // produced by its heavy workgroup
+      const uint64_t bit = (uint64_t)1 << (r - r0);
+      store |= (uint16_t)bit;
+      for (int32_t j = eptr[r]; j < eptr[r + 1]; ++j) {
+        if (keys[j] == kAbsentKey) throw std::invalid_argument("entry key 0xffffffff is reserved for padding");
+        items.push_back(((uint64_t)keys[j] << 16) | bit);
+      }
+    }
+    std::sort(items.begin(), items.end());
+    int64_t n = 0;
+    for (size_t i = 0; i < items.size();) {
+      const uint32_t k = (uint32_t)(items[i] >> 16);
+      uint16_t m = 0;
+      for (; i < items.size() && (uint32_t)(items[i] >> 16) == k; ++i) {
+        const uint16_t bit = (uint16_t)(items[i] & 0xffffu);
+        if (m & bit) throw std::invalid_argument("a row lists the same entry key twice");
+        m |= bit;
+      }
+      out.key.push_back(k);
+      out.mask.push_back(m);
+      ++n;
+    }
+    for (; n % 4; ++n) {
+      out.key.push_back(kAbsentKey);
+      out.mask.push_back(0);
+    }
+    out.count.push_back(n);
+    out.store.push_back(store);
+  }
+}
+
+GroupSet* build_groups(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys, int64_t heavy,
+                       int32_t threads) {
+  if (row_end < row_begin || row_begin < 0) throw std::invalid_argument("bad row range");
+  const int64_t n_groups = (row_end - row_begin + kGroupRows - 1) / kGroupRows;
+  const int64_t T = std::max<int64_t>(1, std::min<int64_t>(threads > 0 ? threads : 1, n_groups));
+  std::vector<GroupPart> parts((size_t)T);
+  std::vector<std::string> errs((size_t)T);
+  std::vector<std::thread> pool;
+  for (int64_t t = 0; t < T; ++t) {
+    pool.emplace_back([&, t]() {
+      try {
+        build_group_part(row_begin, row_end, eptr, keys, heavy, n_groups * t / T, n_groups * (t + 1) / T, parts[(size_t)t]);
+      } catch (const std::exception& e) {
+        errs[(size_t)t] = e.what();
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  for (const auto& e : errs)
+    if (!e.empty()) throw std::invalid_argument(e);
+  auto* gs = new GroupSet();
+  for (auto& part : parts) {
+    for (int64_t c : part.count) gs->off.push_back(gs->off.back() + c);
+    gs->store.insert(gs->store.end(), part.store.begin(), part.store.end());
+    gs->key.insert(gs->key.end(), part.key.begin(), part.key.end());
+    gs->mask.insert(gs->mask.end(), part.mask.begin(), part.mask.end());
+  }
+  return gs;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* grg_host_build_groups(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys,
+                            int64_t heavy_threshold, int32_t threads) {
+  if (!eptr || (!keys && row_end > row_begin)) {
+    g_last_error = "null argument";
+    return nullptr;
+  }
+  try {
+    return build_groups(row_begin, row_end, eptr, keys, heavy_threshold, threads);
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return nullptr;
+  }
+}
+
+int grg_host_groups_sizes(const void* groups, int64_t* sizes) {
+  if (!groups || !sizes) {
+    g_last_error = "null argument";
+    return 1;
+  }
+  const GroupSet* gs = static_cast<const GroupSet*>(groups);
+  sizes[0] = (int64_t)gs->store.size();
+  sizes[1] = (int64_t)gs->key.size();
+  return 0;
+}
+
+int grg_host_groups_copy(const void* groups, int64_t* off, uint16_t* store, uint32_t* key, uint16_t* mask) {
+  if (!groups || !off || !store || !key || !mask) {
+    g_last_error = "null argument";
+    return 1;
+  }
+  const GroupSet* gs = static_cast<const GroupSet*>(groups);
+  std::copy(gs->off.begin(), gs->off.end(), off);
+  std::copy(gs->store.begin(), gs->store.end(), store);
+  std::copy(gs->key.begin(), gs->key.end(), key);
+  std::copy(gs->mask.begin(), gs->mask.end(), mask);
+  return 0;
+}
+
+void grg_host_groups_free(void* groups) { delete static_cast<GroupSet*>(groups); }
+
+}  // extern "C"

@@ -13,7 +13,7 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 14
+ABI_VERSION = 15
 STATIC_SCRATCH_DOUBLES = 1 << 22
 F64 = 0
 F32C = 1
@@ -26,6 +26,8 @@ FLAG_PANEL = 64
 FLAG_XCD_RUNS = 128
 XCD_RUN_SHIFT = 8
 FLAG_GROUP_KEYS = 4096
+FLAG_GROUPS = 8192
+GROUP_ROWS = 16
 TILE_CHUNK_BYTES = 128
 TILE_MAX_STAGED = 1024
 MAX_SEGS = 3
@@ -95,6 +97,11 @@ class GrgHipGatherArgs(ctypes.Structure):
         ("panel_staged", c_int32),
         ("panel_max_entries", c_int32),
         ("panel_max_passes", c_int32),
+        ("grp_off", c_void_p),
+        ("grp_store", c_void_p),
+        ("grp_key", c_void_p),
+        ("grp_mask", c_void_p),
+        ("num_groups", c_int64),
     ]
 
 

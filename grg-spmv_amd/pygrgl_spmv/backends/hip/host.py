@@ -42,6 +42,14 @@ def _lib():
         lib.grg_host_panels_copy.restype = ctypes.c_int
         lib.grg_host_panels_free.argtypes = [c_void_p]
         lib.grg_host_panels_free.restype = None
+        lib.grg_host_build_groups.argtypes = [c_int64, c_int64, c_void_p, c_void_p, c_int64, c_int32]
+        lib.grg_host_build_groups.restype = c_void_p
+        lib.grg_host_groups_sizes.argtypes = [c_void_p, c_void_p]
+        lib.grg_host_groups_sizes.restype = ctypes.c_int
+        lib.grg_host_groups_copy.argtypes = [c_void_p] * 5
+        lib.grg_host_groups_copy.restype = ctypes.c_int
+        lib.grg_host_groups_free.argtypes = [c_void_p]
+        lib.grg_host_groups_free.restype = None
         lib.grg_host_last_error.argtypes = []
         lib.grg_host_last_error.restype = ctypes.c_char_p
         _LIB = lib
@@ -234,3 +242,60 @@ def build_panels(eptr: np.ndarray, keys: np.ndarray, row_begin: int, row_end: in
     return Panels(panel_rows=int(panel_rows), max_staged=int(max_staged), max_pass_entries=max_ent, max_passes=max_passes,
                   pan_pass=pan_pass,
                   pass_src=pass_src, pass_ent=pass_ent, pass_rowoff=pass_rowoff, src_list=src_list, lidx=lidx)
+
+
+GROUP_ROWS = 16  # rows of one group of the row-group launch (the MFMA tile height)
+
+
+@dataclass
+class Groups:
+    """Row groups of one level launch (``grg_host_build_groups``): int64 ``off`` (groups + 1,
+    multiples of 4), uint16 ``store`` (groups), uint32 ``key`` and uint16 ``mask`` (entries)."""
+
+    off: np.ndarray
+    store: np.ndarray
+    key: np.ndarray
+    mask: np.ndarray
+
+    @property
+    def num_groups(self) -> int:
+        return int(self.store.size)
+
+    @property
+    def num_entries(self) -> int:
+        return int(self.key.size)
+
+    @property
+    def nbytes(self) -> int:
+        return int(sum(a.nbytes for a in (self.off, self.store, self.key, self.mask)))
+
+
+def build_groups(eptr: np.ndarray, keys: np.ndarray, row_begin: int, row_end: int, *, heavy_threshold: int = -1,
+                 threads: int = 0) -> Groups:
+    """Row groups of 16 consecutive rows for rows [row_begin, row_end) of entries
+    ``keys[eptr[r]:eptr[r + 1]]``: each group's distinct keys with the mask of rows listing them."""
+    import os
+
+    lib = _lib()
+    eptr = np.ascontiguousarray(eptr, dtype=np.int32)
+    keys = np.ascontiguousarray(keys).view(np.uint32) if np.asarray(keys).dtype.itemsize == 4 else np.ascontiguousarray(keys, dtype=np.uint32)
+    if not 0 <= int(row_begin) <= int(row_end) <= eptr.size - 1:
+        raise ValueError("row range outside eptr")
+    n = int(threads) if threads > 0 else min(16, os.cpu_count() or 1)
+    h = lib.grg_host_build_groups(int(row_begin), int(row_end), eptr.ctypes.data, keys.ctypes.data, int(heavy_threshold), n)
+    if not h:
+        raise RuntimeError(f"grg_host_build_groups failed: {lib.grg_host_last_error().decode()}")
+    try:
+        sz = np.zeros(2, dtype=np.int64)
+        if lib.grg_host_groups_sizes(h, sz.ctypes.data) != 0:
+            raise RuntimeError(f"grg_host_groups_sizes failed: {lib.grg_host_last_error().decode()}")
+        n_groups, n_ent = int(sz[0]), int(sz[1])
+        off = np.empty(n_groups + 1, dtype=np.int64)
+        store = np.empty(max(n_groups, 1), dtype=np.uint16)
+        key = np.empty(max(n_ent, 1), dtype=np.uint32)
+        mask = np.empty(max(n_ent, 1), dtype=np.uint16)
+        if lib.grg_host_groups_copy(h, off.ctypes.data, store.ctypes.data, key.ctypes.data, mask.ctypes.data) != 0:
+            raise RuntimeError(f"grg_host_groups_copy failed: {lib.grg_host_last_error().decode()}")
+    finally:
+        lib.grg_host_groups_free(h)
+    return Groups(off=off, store=store[:n_groups], key=key[:n_ent], mask=mask[:n_ent])

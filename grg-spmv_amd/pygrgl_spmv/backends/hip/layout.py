@@ -254,11 +254,12 @@ def schedule_capacity_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeReq
 
 PANEL_ROWS = 64       # rows of one panel of the panel launches
 PANEL_STAGED = 128    # source rows staged per panel pass
+GROUP_ROWS = 16       # rows of one group of the row-group launches (FP64 values)
 
 
-def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequirements) -> int:
-    """Upper bound of the tile tables of the tiled level launches and of the panels
-    of the panel launches.
+def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequirements, dtype=np.float64) -> int:
+    """Upper bound of the tile tables of the tiled level launches, of the panels
+    of the panel launches and (FP64 values) of the row groups of the row-group launches.
 
     Per sweep configuration (up; down; down with missing-data input): a uint16
     staged index per gather entry (plus 16 bytes of padding), at most one uint32
@@ -285,7 +286,15 @@ def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequiremen
     for e, _ in entries:
         passes = e // PANEL_STAGED + panels
         pan += 5 * e + 80 + 4 * (panels + 1) + (8 + 2 * (PANEL_ROWS + 1)) * (passes + 1)
-    return int(tiles + pan)
+    # row groups (no more than one per GROUP_ROWS rows plus one per level): at most one
+    # uint32 key and uint16 mask per entry plus three padding entries per group, an
+    # int64 offset and a uint16 store mask per group
+    grp = 0
+    if np.dtype(dtype) == np.float64:
+        groups = N // GROUP_ROWS + L + 1
+        for e, _ in entries:
+            grp += 6 * (e + 3 * groups) + 10 * (groups + 1) + 32
+    return int(tiles + pan + grp)
 
 
 def plan_hip_layout(
@@ -343,7 +352,7 @@ def plan_hip_layout(
         if requirements.need_init_xtx and s.has_individual_coals:
             xtx_bytes += s.num_nodes * dtype.itemsize
         if tiles:
-            tile_bytes += tile_bytes_for(s, pair, requirements)
+            tile_bytes += tile_bytes_for(s, pair, requirements, dtype)
         cap = schedule_capacity_for(s, pair, requirements)
         heavy_caps.append(cap)
         heavy_bytes += 4 * cap

@@ -74,8 +74,11 @@ TILE_MAX_ROWS = 256     # rows of one tile
 TILE_MAX_ENTRIES = 2048  # gather entries of one tile
 PANEL_MIN_MEAN = 32.0   # levels whose light rows average fewer entries get no panels
 PANEL_MIN_ROWS = 1024   # ... nor levels with fewer rows
-PANEL_MAX_PASS_BYTES = 16384  # entry positions of one pass (with 32 bytes of slack) the panel kernel stages
+PANEL_MAX_PASS_BYTES = 4096   # entry positions of one pass (with 32 bytes of slack) the panel kernel stages
 PANEL_MAX_PASSES = 2048       # passes of one panel (their entry offsets sit in LDS)
+GROUP_MIN_MEAN = 3.0    # levels whose light rows average fewer entries get no row groups
+GROUP_MIN_ROWS = 256    # ... nor levels with fewer rows
+GROUP_MAX_FILL = 0.9    # ... nor levels whose groups keep more than this share of the entries
 DEGREE_WINDOW = 4096    # nodes per degree-sorted run of the device node order
 DEGREE_MIN_MEAN = 6.0
 # Run lengths (log2 of light blocks) the tuner tries for GRG_HIP_FLAG_XCD_RUNS.
@@ -135,17 +138,18 @@ def tile_lds_bytes(dtype_code: int, cpl: int, kl: int, staged_rows: int, max_row
 
 def panel_lds_bytes(dtype_code: int, cpl: int, kl: int, staged_rows: int, panel_rows: int, max_entries: int,
                     max_passes: int) -> int:
-    """LDS of one panel workgroup: the panel's int32 pass entry offsets, one pass's
-    staged rows (as in a tile), its uint16 row offsets and uint8 entry positions with
-    32 bytes of slack (each part rounded to 16 bytes), and the 2 KiB reduction buffer."""
+    """LDS of one panel workgroup: the panel's int32 pass source and entry offsets, then
+    two pass buffers, each one pass's staged rows (as in a tile), its uint16 row offsets
+    and PANEL_MAX_PASS_BYTES of entry positions (each part rounded to 16 bytes), and the
+    2 KiB reduction buffer. (``max_entries`` is bounded by PANEL_MAX_PASS_BYTES.)"""
     elem = 8 if dtype_code == ffi.F64 else 4
     pad = cpl if cpl * elem >= 16 else 16 // elem
 
     def r16(b):
         return (int(b) + 15) // 16 * 16
 
-    return (r16((int(max_passes) + 1) * 4) + r16(int(staged_rows) * (kl * cpl + pad) * elem) + r16((int(panel_rows) + 1) * 2)
-            + r16(int(max_entries) + 32) + 8 * 4 * 64)
+    buffer = r16(int(staged_rows) * (kl * cpl + pad) * elem) + r16((int(panel_rows) + 1) * 2) + PANEL_MAX_PASS_BYTES
+    return r16(2 * (int(max_passes) + 1) * 4) + 2 * buffer + 8 * 4 * 64
 
 
 @dataclass(frozen=True)
@@ -236,6 +240,7 @@ class _HipArtifact:
     max_items: int = 0                                # largest split-item count of one level launch
     tiles: dict = field(default_factory=dict)         # config -> per-level tile offsets and LDS maxima
     panels: dict = field(default_factory=dict)        # (config, level) -> (panels, most entries of a pass, most passes of a panel)
+    groups: dict = field(default_factory=dict)        # (config, level) -> (row groups, group entries)
 
 
 @dataclass
@@ -431,8 +436,12 @@ class HipBatchedSweep:
                     for i in range(n):
                         a = ffi.GrgHipGatherArgs()
                         ctypes.pointer(a)[0] = arr[i]
+                        if a.flags & ffi.FLAG_GROUPS:
+                            # job tables run the per-row light path: the row-group shape's 16 column
+                            # lanes become one row's lane group, one edge lane, 8 gathers in flight
+                            a.es, a.unroll, a.rows_per_group = 1, 8, 1
                         a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_PANEL | ffi.FLAG_XCD_RUNS | XCD_RUN_BITS |
-                                     ffi.FLAG_GROUP_KEYS)
+                                     ffi.FLAG_GROUP_KEYS | ffi.FLAG_GROUPS)
                         launches.append(a)
                         phase_of.append(p)
                         p += 1
@@ -487,7 +496,9 @@ class HipRuntime:
                  tile_max_staged: int = TILE_MAX_STAGED, tile_max_rows: int = TILE_MAX_ROWS,
                  tile_max_entries: int = TILE_MAX_ENTRIES, degree_window: int = DEGREE_WINDOW,
                  degree_min_mean: float = DEGREE_MIN_MEAN, panels: bool = False, panel_min_mean: float = PANEL_MIN_MEAN,
-                 panel_min_rows: int = PANEL_MIN_ROWS, tune_cache="default") -> None:
+                 panel_min_rows: int = PANEL_MIN_ROWS, groups: bool = True, group_min_mean: float = GROUP_MIN_MEAN,
+                 group_min_rows: int = GROUP_MIN_ROWS, group_max_fill: float = GROUP_MAX_FILL,
+                 tune_cache="default") -> None:
         """``autotune``: time the admissible launch shapes of every level launch when an
         op is prepared (otherwise a fixed default shape). ``split_min`` /
         ``split_chunk``: rows of gather degree above ``split_min`` are summed in
@@ -507,7 +518,11 @@ class HipRuntime:
         on MI355X the per-row forms are faster at every level measured so far): with the
         tile tables, also build the panels of the panel launches (levels whose light rows
         average at least ``panel_min_mean`` entries, with at least ``panel_min_rows``
-        rows) and let the tuner choose them. ``tune_cache``: directory of the on-disk cache of
+        rows) and let the tuner choose them. ``groups`` (FP64 values, with the tile tables):
+        build the row groups of the row-group launches (levels with at least ``group_min_rows``
+        rows whose light rows average at least ``group_min_mean`` entries, kept when a level's
+        group entries are at most ``group_max_fill`` of its entries) and let the tuner choose
+        them. ``tune_cache``: directory of the on-disk cache of
         tuned launch shapes (``tunecache``; "default": ``$GRG_HIP_TUNE_CACHE`` or
         ``~/.cache/pygrgl_spmv/hip_tune``; None: no disk cache)."""
         if int(degree_window) < 0:
@@ -535,6 +550,10 @@ class HipRuntime:
         self.tile_max_rows = int(tile_max_rows)
         self.tile_max_entries = int(tile_max_entries)
         self.use_panels = bool(panels)
+        self.use_groups = bool(groups)
+        self.group_min_mean = float(group_min_mean)
+        self.group_min_rows = int(group_min_rows)
+        self.group_max_fill = float(group_max_fill)
         self.panel_min_mean = float(panel_min_mean)
         self.panel_min_rows = int(panel_min_rows)
         self.degree_window = int(degree_window)
@@ -890,8 +909,9 @@ class HipRuntime:
             if cursor > cap:
                 raise RuntimeError(f"internal error: {cursor} schedule entries exceed the planned capacity {cap}")
             D["sched"] = self._to_dev(np.concatenate(tables).astype(np.int32) if tables else np.zeros(1, np.int32))
-            if lay.tiles and self.use_tiles:
-                # tile tables of the LDS-staged level launches, one set per sweep configuration
+            if lay.tiles and (self.use_tiles or self.use_groups):
+                # tile tables of the LDS-staged level launches and row groups of the row-group
+                # launches, one set per sweep configuration
                 seg_host = {}
                 if "up_ptr" in D:
                     seg_host["up"] = [(D["up_ptr"].cpu().numpy(), D["up_idx"].cpu().numpy(), N)]
@@ -904,22 +924,32 @@ class HipRuntime:
                     if name not in configs:
                         continue
                     heavy = [art.sched[(name, l)].heavy_threshold for l in range(L)]
-                    t = build_tiles(off, segs_h, heavy, max_staged=self.tile_max_staged, max_rows=self.tile_max_rows,
-                                    max_entries=self.tile_max_entries)
-                    D[f"{name}_tile_row"] = self._to_dev(t.tile_row)
-                    D[f"{name}_tile_src"] = self._to_dev(t.tile_src)
-                    D[f"{name}_src_list"] = self._to_dev(t.src_list.view(np.int32))
-                    D[f"{name}_eptr"] = self._to_dev(t.eptr)
-                    D[f"{name}_lidx"] = self._to_dev(t.lidx.view(np.int16))
+                    if self.use_tiles:
+                        t = build_tiles(off, segs_h, heavy, max_staged=self.tile_max_staged, max_rows=self.tile_max_rows,
+                                        max_entries=self.tile_max_entries)
+                        D[f"{name}_tile_row"] = self._to_dev(t.tile_row)
+                        D[f"{name}_tile_src"] = self._to_dev(t.tile_src)
+                        D[f"{name}_src_list"] = self._to_dev(t.src_list.view(np.int32))
+                        D[f"{name}_lidx"] = self._to_dev(t.lidx.view(np.int16))
+                        art.tiles[name] = (t.level_tile_off.copy(), t.level_max_staged.copy(), t.level_max_rows.copy(),
+                                           t.level_max_entries.copy())
+                        eptr_h = t.eptr
+                        del t
+                    else:
+                        # merged row pointers: each row's entries over all segments, in segment order
+                        deg = sum(np.diff(np.asarray(p_, dtype=np.int64)) for p_, _, _ in segs_h)
+                        eptr64 = np.zeros(N + 1, dtype=np.int64)
+                        np.cumsum(deg, out=eptr64[1:])
+                        eptr_h = _i32(eptr64, f"{name} merged row pointers")
+                    D[f"{name}_eptr"] = self._to_dev(eptr_h)
+                    keys = merged_keys(eptr_h, segs_h) if len(segs_h) > 1 else segs_h[0][1]
                     if len(segs_h) > 1:
-                        D[f"{name}_ekey"] = self._to_dev(merged_keys(t.eptr, segs_h).view(np.int32))
-                    art.tiles[name] = (t.level_tile_off.copy(), t.level_max_staged.copy(), t.level_max_rows.copy(),
-                                       t.level_max_entries.copy())
-                    if self.use_panels:
-                        keys = merged_keys(t.eptr, segs_h) if len(segs_h) > 1 else segs_h[0][1]
-                        self._build_panels(art, name, t.eptr, keys, off)
-                        del keys
-                    del t
+                        D[f"{name}_ekey"] = self._to_dev(keys.view(np.int32))
+                    if self.use_panels and self.use_tiles:
+                        self._build_panels(art, name, eptr_h, keys, off)
+                    if self.use_groups and self._dtype_code == ffi.F64:
+                        self._build_groups(art, name, eptr_h, keys, off)
+                    del keys, eptr_h
                 del seg_host
             # streamed ops with slot-reuse links, and their events
             art.up_stream = relink_stream_dependencies(up_stream, direction=Direction.UP)
@@ -953,7 +983,9 @@ class HipRuntime:
     def _shape_admissible(self, a: ffi.GrgHipGatherArgs, shape, lds) -> bool:
         """Whether a cached shape is one the kernel accepts for this launch."""
         cpl, kl, es, unroll, rows, fl = shape
-        if (cpl, kl) not in column_shapes(a.dtype, a.k, lds) and not (fl & (ffi.FLAG_TILED | ffi.FLAG_PANEL)):
+        if (cpl, kl) not in column_shapes(a.dtype, a.k, lds) and not (fl & (ffi.FLAG_TILED | ffi.FLAG_PANEL | ffi.FLAG_GROUPS)):
+            return False
+        if fl & ffi.FLAG_GROUPS and (not a.grp_off or a.dtype != ffi.F64 or kl * cpl != ffi.GROUP_ROWS):
             return False
         if cpl < 1 or a.k % cpl or any(int(ld) % cpl for ld in lds) or kl < 1 or kl & (kl - 1) or es < 1 or es & (es - 1):
             return False
@@ -990,6 +1022,41 @@ class HipRuntime:
             D[f"{tag}_lidx"] = self._to_dev(pn.lidx.view(np.int8))
             art.panels[(cfg, l)] = (pn.num_panels, pn.max_pass_entries, pn.max_passes)
 
+    def _build_groups(self, art: _HipArtifact, cfg: str, eptr: np.ndarray, keys: np.ndarray, off: np.ndarray) -> None:
+        """Row groups (``grg_host_build_groups``) of the level launches of sweep configuration
+        ``cfg`` whose light rows average at least ``group_min_mean`` entries, kept where
+        neighbouring rows share sources (fewer group entries than ``group_max_fill`` of the
+        rows' entries)."""
+        from pygrgl_spmv.backends.hip.host import build_groups
+
+        D = art.dev
+        for l in range(off.size - 1):
+            lo, hi = int(off[l]), int(off[l + 1])
+            sched = art.sched.get((cfg, l))
+            if sched is None or (cfg == "up" and l == 0) or hi - lo < max(1, self.group_min_rows):
+                continue
+            if sched.mean_degree < self.group_min_mean:
+                continue
+            gr = build_groups(eptr, keys, lo, hi, heavy_threshold=int(sched.heavy_threshold))
+            if gr.num_entries > self.group_max_fill * max(1, int(eptr[hi]) - int(eptr[lo])):
+                continue
+            tag = f"{cfg}_grp{l}"
+            D[f"{tag}_off"] = self._to_dev(gr.off)
+            D[f"{tag}_store"] = self._to_dev(gr.store.view(np.int16))
+            D[f"{tag}_key"] = self._to_dev(gr.key.view(np.int32))
+            D[f"{tag}_mask"] = self._to_dev(gr.mask.view(np.int16))
+            art.groups[(cfg, l)] = (gr.num_groups, gr.num_entries)
+
+    def _group_fields(self, art: _HipArtifact, cfg: str, level: int):
+        """Row-group fields of one level launch of sweep configuration ``cfg``, or None."""
+        meta = art.groups.get((cfg, level))
+        if meta is None:
+            return None
+        D = art.dev
+        tag = f"{cfg}_grp{level}"
+        return {"grp_off": D[f"{tag}_off"].data_ptr(), "grp_store": D[f"{tag}_store"].data_ptr(),
+                "grp_key": D[f"{tag}_key"].data_ptr(), "grp_mask": D[f"{tag}_mask"].data_ptr(), "num_groups": int(meta[0])}
+
     def _panel_fields(self, art: _HipArtifact, cfg: str, level: int):
         """Panel fields of one level launch of sweep configuration ``cfg``, or None."""
         meta = art.panels.get((cfg, level))
@@ -1021,10 +1088,11 @@ class HipRuntime:
 
     def _args(self, *, rows, k, out, ld_out, segs=(), maps=(), row_scale=None, col_vec=None, accumulate=False,
               sched: _LevelSchedule | None = None, sched_base: int = 0, mean_degree: float = 1.0,
-              out_base=0, tiles: dict | None = None, panels: dict | None = None, partials=None) -> ffi.GrgHipGatherArgs:
+              out_base=0, tiles: dict | None = None, panels: dict | None = None, groups: dict | None = None,
+              partials=None) -> ffi.GrgHipGatherArgs:
         """Launch descriptor for rows [rows); ``sched`` adds the level's heavy rows,
         ``tiles`` (from ``_tile_fields``) the level's tile tables, ``panels`` (from
-        ``_panel_fields``) its panels."""
+        ``_panel_fields``) its panels, ``groups`` (from ``_group_fields``) its row groups."""
         a = ffi.GrgHipGatherArgs()
         a.dtype = self._dtype_code
         a.accumulate = 1 if accumulate else 0
@@ -1052,6 +1120,9 @@ class HipRuntime:
         if panels:
             for name, value in panels.items():
                 setattr(a, name, int(value))
+        if groups:
+            for name, value in groups.items():
+                setattr(a, name, int(value))
         for name, value in self.fixed_shape.items():
             setattr(a, name, int(value))
         if (a.flags & ffi.FLAG_MERGED) and not a.ekey:
@@ -1074,6 +1145,15 @@ class HipRuntime:
                 a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_MERGED)
                 a.kl = min(a.kl, group)
                 a.es = group // a.kl
+        if a.flags & ffi.FLAG_GROUPS:
+            # a row-group launch needs the level's groups (FP64) and one 16-column chunk
+            if not groups or a.dtype != ffi.F64 or a.cpl not in (1, 2, 4):
+                a.flags &= ~ffi.FLAG_GROUPS
+            else:
+                a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_PANEL | ffi.FLAG_STREAM | ffi.FLAG_MERGED | ffi.FLAG_GROUP_KEYS)
+                a.kl, a.es, a.rows_per_group = ffi.GROUP_ROWS // a.cpl, 1, 1
+                if a.unroll not in (4, 8, 16):
+                    a.unroll = 8
         if a.flags & ffi.FLAG_TILED:
             if not tiles:
                 a.flags &= ~ffi.FLAG_TILED
@@ -1082,7 +1162,7 @@ class HipRuntime:
                     a.kl //= 2
         if a.flags & (ffi.FLAG_XCD_REMAP | ffi.FLAG_PANEL | ffi.FLAG_STREAM):
             a.flags &= ~(ffi.FLAG_XCD_RUNS | XCD_RUN_BITS)  # runs apply to the light path and to tiles
-        if a.kl * a.es > 64 and not a.flags & ffi.FLAG_PANEL:
+        if a.kl * a.es > 64 and not a.flags & (ffi.FLAG_PANEL | ffi.FLAG_GROUPS):
             a.es = 64 // a.kl
         if (a.flags & ffi.FLAG_GROUP_KEYS) and ((not a.flags & ffi.FLAG_MERGED and a.nseg != 1) or a.rows_per_group > 1
                                                 or a.flags & (ffi.FLAG_TILED | ffi.FLAG_PANEL | ffi.FLAG_STREAM)):
@@ -1202,7 +1282,8 @@ class HipRuntime:
                         for klp in sorted({min(kl, group), max(1, min(kl, group) // 2)}):
                             if a.panel_staged * klp > 1024 or panel_lds_bytes(a.dtype, cpl, klp, a.panel_staged, a.panel_rows, a.panel_max_entries, a.panel_max_passes) > TILE_LDS_LIMIT:
                                 continue
-                            time_shape(cpl, klp, group // klp, 8, 1, ffi.FLAG_PANEL)
+                            for u in (4, 8):  # two or three passes in flight
+                                time_shape(cpl, klp, group // klp, u, 1, ffi.FLAG_PANEL)
                     if len(timings) > n_before:
                         _, cpl, kl, es, unroll, rows, _ = min(timings[n_before:])
                         time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_PANEL | ffi.FLAG_XCD_REMAP)
@@ -1223,6 +1304,19 @@ class HipRuntime:
                         time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_TILED | ffi.FLAG_XCD_REMAP)
                         for log_run in XCD_RUN_LOGS:
                             time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_TILED | xcd_runs_flags(log_run))
+                if a.grp_off and a.dtype == ffi.F64:
+                    # the row-group form: one wave per 16 consecutive rows, the group's distinct
+                    # sources gathered once and summed by MFMA (16-column chunks)
+                    n_before = len(timings)
+                    cpls = [c for c, _ in column_shapes(a.dtype, a.k, lds) if c in (1, 2, 4)]
+                    for cpl in cpls:
+                        for u in (4, 8, 16):
+                            time_shape(cpl, ffi.GROUP_ROWS // cpl, 1, u, 1, ffi.FLAG_GROUPS)
+                    if len(timings) > n_before:
+                        _, cpl, kl, es, unroll, rows, _ = min(timings[n_before:])
+                        time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_GROUPS | ffi.FLAG_XCD_REMAP)
+                        for log_run in XCD_RUN_LOGS:
+                            time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_GROUPS | xcd_runs_flags(log_run))
                 top = min(timings)
                 finalists = []
                 for t in sorted(timings):
@@ -1322,7 +1416,8 @@ class HipRuntime:
             level_args = self._args(rows=(lo, hi), k=k, out=V, ld_out=ldv, segs=segs, maps=maps,
                                     row_scale=row_scale_init, col_vec=col_vec_init, sched=sched, sched_base=sched_base,
                                     tiles=self._tile_fields(art, cfg, l) if segs else None,
-                                    panels=self._panel_fields(art, cfg, l) if segs else None, partials=partials)
+                                    panels=self._panel_fields(art, cfg, l) if segs else None,
+                                    groups=self._group_fields(art, cfg, l) if segs else None, partials=partials)
             if segs:
                 self._tune(level_args, (art.path, cfg, l, k))
             if sched is not None and sched.num_items:

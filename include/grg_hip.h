@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 14
+#define GRG_HIP_ABI_VERSION 15
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -85,6 +85,11 @@ enum GrgHipDtype {
  * for bit. Not for tiled, panel, streamed or partials launches. No reference
  * counterpart. */
 #define GRG_HIP_FLAG_GROUP_KEYS 4096
+/* Row-group launch: see the group fields of GrgHipGatherArgs. FP64 values only.
+ * Combines with GRG_HIP_FLAG_XCD_REMAP or GRG_HIP_FLAG_XCD_RUNS (over the
+ * launch's group workgroups), not with the other forms. No reference
+ * counterpart. */
+#define GRG_HIP_FLAG_GROUPS 8192
 /* Bytes of one staged row (kl * cpl columns) in a tiled launch, at most. */
 #define GRG_HIP_TILE_CHUNK_BYTES 128
 /* Staged rows of one tile, at most (uint16 LDS indices). */
@@ -215,6 +220,27 @@ typedef struct GrgHipGatherArgs {
   int32_t panel_staged;
   int32_t panel_max_entries;
   int32_t panel_max_passes; /* most passes of one panel (sizes the LDS) */
+  /* Row-group launch (GRG_HIP_FLAG_GROUPS; no reference counterpart: rows that
+   * share sources gathered once per group). Rows [row_begin, row_end) are cut
+   * into num_groups groups of 16 consecutive rows (group g: rows row_begin +
+   * 16 g + [0, 16), cut at row_end), one workgroup each. Group g's entries are
+   * [grp_off[g], grp_off[g + 1]) (multiples of 4): grp_key[e] is a source
+   * ((segment << 30) | source row, 0xffffffff for padding) and bit i of
+   * grp_mask[e] says that row 16 g + i sums it. A row's value is the sum of the
+   * sources whose mask has its bit, plus its row terms; the group stores the
+   * rows whose bit is set in grp_store[g]. The sums run four entries at a time
+   * as one FP64 16 x 16 x 4 MFMA per 16-column chunk (A = the 0/1 mask bits,
+   * B = the four source rows), so every source row is read once per group.
+   * Heavy rows (as above) hold no group entries and are produced by the heavy
+   * workgroups after the groups: grid = num_groups + num_heavy by ceil(k / 16)
+   * column chunks; kl * cpl must be 16 (the heavy rows' column chunk). The
+   * workgroup's four waves take interleaved rounds of four entries (unroll
+   * rounds in flight per wave) and add their tiles in wave order. */
+  const int64_t* grp_off;
+  const uint16_t* grp_store;
+  const uint32_t* grp_key;
+  const uint16_t* grp_mask;
+  int64_t num_groups;
 } GrgHipGatherArgs;
 
 /* One row-gather launch (see above). Replaces one level of the reference's

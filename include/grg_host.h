@@ -113,6 +113,32 @@ int grg_host_panels_copy(const void* panels, int32_t* pan_pass, int32_t* pass_sr
                          uint32_t* src_list, uint8_t* lidx);
 void grg_host_panels_free(void* panels);
 
+/* Row groups of the row-group launch of libgrg_hip.so (GRG_HIP_FLAG_GROUPS; see
+ * the group fields of GrgHipGatherArgs in grg_hip.h) for rows [row_begin,
+ * row_end) of one level. No reference counterpart.
+ *
+ * Row r's entries are keys[eptr[r] .. eptr[r + 1]) (merged entry keys
+ * (segment << 30) | source row, or plain source rows of one segment), each key
+ * at most once per row (a repeated key is an error) and none equal to
+ * 0xffffffff. Group g is rows row_begin + 16 g + [0, 16), cut at row_end. Rows
+ * of degree > heavy_threshold (when >= 0) are left to their heavy workgroups:
+ * they contribute no entries and are not stored by the group. A group lists
+ * the distinct keys of its other rows, ascending, each with a 16-bit mask of
+ * the rows (bit i = row 16 g + i) that list it, padded with (0xffffffff, 0)
+ * entries to a multiple of 4. threads > 1 builds on that many host threads.
+ *
+ * Returns an opaque handle (NULL on error, see grg_host_last_error), read with
+ * grg_host_groups_sizes / grg_host_groups_copy and released with
+ * grg_host_groups_free. sizes[2] = {groups, entries}. Copied arrays:
+ *   off   [groups + 1]: group g's entries are [off[g], off[g + 1]) (multiples of 4);
+ *   store [groups]: bit i set when row 16 g + i is produced by the group;
+ *   key   [entries], mask [entries]: the group entries. */
+void* grg_host_build_groups(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys,
+                            int64_t heavy_threshold, int32_t threads);
+int grg_host_groups_sizes(const void* groups, int64_t* sizes);
+int grg_host_groups_copy(const void* groups, int64_t* off, uint16_t* store, uint32_t* key, uint16_t* mask);
+void grg_host_groups_free(void* groups);
+
 const char* grg_host_last_error(void);
 
 #ifdef __cplusplus

@@ -369,6 +369,74 @@ def test_tuner_may_choose_tiles(art_a, oracle_a):
 
 
 # ---------------------------------------------------------------------------
+# row-group launches (FP64, MFMA)
+# ---------------------------------------------------------------------------
+
+
+def _group_runtime(layout, shape, **kw):
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    # groups on every level launch with segments, whatever their sharing
+    return HipRuntime(layout, fixed_shape=shape, group_min_mean=0.0, group_min_rows=1, group_max_fill=100.0, **kw)
+
+
+@pytest.mark.parametrize("cpl,unroll", [(1, 4), (2, 8), (4, 16)])
+@pytest.mark.parametrize("order", ["plain", "remap", "runs"])
+def test_row_group_launches(art_a, oracle_a, cpl, unroll, order):
+    """Every level launch of both sweeps as a row-group launch (16-row groups summed by FP64
+    MFMA, 16-column chunks), heavy and split rows beside it, in each XCD block order:
+    exact for +-1 inputs, within 1e-12 for N(0, 1), with missing data, by-individual
+    columns and the init modes; k from 1 to 33 covers partial and several column chunks."""
+    from pygrgl_spmv.backends.hip import ffi
+    from pygrgl_spmv.backends.hip.runtime import xcd_runs_flags
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=36, max_k_down=36))
+    extra = {"plain": 0, "remap": ffi.FLAG_XCD_REMAP, "runs": xcd_runs_flags(2)}[order]
+    shape = {"unroll": unroll, "rows_per_group": 1, "es": 1, "cpl": cpl, "kl": 16 // cpl, "flags": ffi.FLAG_GROUPS | extra}
+    with _group_runtime(layout, shape, split_min=40, split_chunk=16) as rt:
+        (grg,) = rt.grgs
+        assert rt._artifacts[0].groups, "row groups must be built"
+        rng = np.random.default_rng(3950 + cpl + unroll)
+        for k in (1, 4, 16, 20, 36) if cpl == 4 else (1, 2, 16, 33):
+            if k % cpl:
+                continue
+            for direction in ("up", "down"):
+                x = rng.choice(np.array([-1.0, 1.0]), size=(k, _cols(oracle_a, direction)))
+                np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
+        y = rng.standard_normal((4, oracle_a.num_mutations))
+        miss = rng.standard_normal((4, oracle_a.num_mutations))
+        _check(grg.matmul(y, "down", miss=miss.copy()), O.matmul(oracle_a, y, "down", miss=miss.copy()))
+        x = rng.standard_normal((4, oracle_a.num_individuals))
+        _check(grg.matmul(x, "up", by_individual=True, init="xtx"), O.matmul(oracle_a, x, "up", by_individual=True, init="xtx"))
+        y = rng.standard_normal((4, oracle_a.num_mutations))
+        init = rng.standard_normal((4, oracle_a.num_nodes))
+        _check(grg.matmul(y, "down", by_individual=True, init=init), O.matmul(oracle_a, y, "down", by_individual=True, init=init))
+
+
+def test_row_groups_cover_levels_and_tuner_may_choose_them(art_a, oracle_a):
+    """Groups are built for the FP64 level launches (not for FP32c runtimes), every group
+    entry set and store mask is consistent with the level's rows, and a tuned runtime (the
+    row-group form among its candidates) matches the oracle."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    for dtype in (np.float64, np.float32):
+        layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
+        with HipRuntime(layout, group_min_mean=0.0, group_min_rows=1, group_max_fill=100.0) as rt:
+            art = rt._artifacts[0]
+            assert bool(art.groups) == (dtype == np.float64)
+            for (cfg, level), (n_groups, n_ent) in art.groups.items():
+                rows = int(art.level_offsets[level + 1] - art.level_offsets[level])
+                assert n_groups == -(-rows // 16) and n_ent % 4 == 0
+            (grg,) = rt.grgs
+            rng = np.random.default_rng(3960)
+            x = rng.standard_normal((16, oracle_a.num_samples)).astype(dtype)
+            y = rng.standard_normal((16, oracle_a.num_mutations)).astype(dtype)
+            tol = FP64_TOL if dtype == np.float64 else FP32_TOL
+            assert O.normwise_rel_err(grg.matmul(x, "up"), O.matmul(oracle_a, x.astype(np.float64), "up")) <= tol
+            assert O.normwise_rel_err(grg.matmul(y, "down"), O.matmul(oracle_a, y.astype(np.float64), "down")) <= tol
+
+
+# ---------------------------------------------------------------------------
 # streamed blocks through the pinned ring
 # ---------------------------------------------------------------------------
 
