@@ -1483,6 +1483,27 @@ class HipRuntime:
 
     # ---- execution -------------------------------------------------------
 
+    def _copy_block_to_slot(self, step: _StreamStep, prev_compute_done, copy_done) -> None:
+        """H2D copy of a streamed block into its ring slot on the slot's copy stream,
+        after the previous compute that read the slot (``prev_compute_done``, None for
+        the slot's first use in the sweep); ``copy_done`` is recorded after it. The
+        counterpart of the reference's ``_copy_host_block_to_slot``
+        (``triton/backend.py:958-968``)."""
+        torch = _torch()
+        cs = self._slot_copy_streams[step.slot]
+        with torch.cuda.stream(cs):
+            if prev_compute_done is not None:
+                cs.wait_event(prev_compute_done)
+            n_idx = int(step.host_indices.numel())
+            n_ptr = int(step.host_indptr.numel())
+            self._slot_indices[step.slot][:n_idx].copy_(step.host_indices, non_blocking=True)
+            self._slot_indptr[step.slot][:n_ptr].copy_(step.host_indptr, non_blocking=True)
+            copy_done.record(cs)
+
+    def _launch_streamed_block(self, args: ffi.GrgHipGatherArgs, stream_ptr: int) -> None:
+        """The accumulate launch of one streamed block (after its slot copy)."""
+        self._lib.gather(args, stream_ptr)
+
     def _execute(self, art: _HipArtifact, spec: _CudaMatmulSpec, items) -> None:
         torch = _torch()
         with self._caller_root_scope():
@@ -1506,19 +1527,14 @@ class HipRuntime:
                     self._lib.gather_batch(it[1], it[2], root_ptr)
                     continue
                 _, step, args, (copy_done, compute_done), _key = it
-                cs = self._slot_copy_streams[step.slot]
-                with torch.cuda.stream(cs):
-                    if step.prev_in_slot is not None:
-                        pl, pi = step.prev_in_slot
-                        ev_list = art.up_events if spec.direction == Direction.UP else art.down_events
-                        cs.wait_event(ev_list[pl][pi][1])
-                    n_idx = int(step.host_indices.numel())
-                    n_ptr = int(step.host_indptr.numel())
-                    self._slot_indices[step.slot][:n_idx].copy_(step.host_indices, non_blocking=True)
-                    self._slot_indptr[step.slot][:n_ptr].copy_(step.host_indptr, non_blocking=True)
-                    copy_done.record(cs)
+                prev = None
+                if step.prev_in_slot is not None:
+                    pl, pi = step.prev_in_slot
+                    ev_list = art.up_events if spec.direction == Direction.UP else art.down_events
+                    prev = ev_list[pl][pi][1]
+                self._copy_block_to_slot(step, prev, copy_done)
                 root.wait_event(copy_done)
-                self._lib.gather(args, root_ptr)
+                self._launch_streamed_block(args, root_ptr)
                 with torch.cuda.stream(root):
                     compute_done.record(root)
             if timer is not None:

@@ -495,6 +495,71 @@ def test_streamed_band_closed_form(artifact_dir):
         np.testing.assert_array_equal(grg.matmul(x, "down"), O.band_expected_down(x.T, shifts=(0, bw, 2 * bw), bandwidth=bw).T)
 
 
+def _overlaps(compute, copies) -> bool:
+    """True when some timed slot copy ran while a timed streamed launch was in flight."""
+    for c0, c1 in compute:
+        for h0, h1 in copies:
+            if h0.elapsed_time(c1) > 0.0 and c0.elapsed_time(h1) > 0.0:
+                return True
+    return False
+
+
+@pytest.mark.parametrize("ring", [1, 2])
+def test_ring_copies_overlap_compute(art_a, oracle_a, monkeypatch, ring):
+    """Slot copies run on their own streams: with one slot each copy waits for the launch
+    that read the slot before, so no copy overlaps a launch; with two slots the copy of
+    the next block runs under the current launch. Each streamed launch is followed by a
+    device spin so that a launch window is long enough to see; the results stay exact.
+    Counterpart: ``runtime/test_triton.py:363-441`` (timed copy and launch wrappers)."""
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=2, max_k_down=2),
+                              ring_buffer_size=ring, allow_residency=False)
+    assert layout.allocated_ring_buffer_size == ring
+    state = {"active": True, "compute": [], "copy": []}
+    launch, copy = HipRuntime._launch_streamed_block, HipRuntime._copy_block_to_slot
+
+    def timed_launch(self, args, stream_ptr):
+        if not state["active"]:
+            return launch(self, args, stream_ptr)
+        root = self._root_stream
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(root)
+        launch(self, args, stream_ptr)
+        with torch.cuda.stream(root):
+            torch.cuda._sleep(5_000_000)
+        t1.record(root)
+        state["compute"].append((t0, t1))
+
+    def timed_copy(self, step, prev_compute_done, copy_done):
+        if not state["active"]:
+            return copy(self, step, prev_compute_done, copy_done)
+        cs = self._slot_copy_streams[step.slot]
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        if prev_compute_done is not None:
+            cs.wait_event(prev_compute_done)
+        t0.record(cs)
+        copy(self, step, None, copy_done)
+        t1.record(cs)
+        state["copy"].append((t0, t1))
+
+    monkeypatch.setattr(HipRuntime, "_launch_streamed_block", timed_launch)
+    monkeypatch.setattr(HipRuntime, "_copy_block_to_slot", timed_copy)
+    rng = np.random.default_rng(3700 + ring)
+    with HipRuntime(layout) as rt:
+        (grg,) = rt.grgs
+        for i, direction in enumerate(("up", "down", "up")):
+            x = rng.choice(np.array([-1.0, 1.0]), size=(2, _cols(oracle_a, direction)))
+            np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
+            torch.cuda.synchronize()
+            if i == 0:
+                assert len(state["compute"]) >= 3 and len(state["copy"]) == len(state["compute"])
+                assert _overlaps(state["compute"], state["copy"]) == (ring > 1)
+                state["active"] = False
+
+
 # ---------------------------------------------------------------------------
 # hipGraph capture of prepared op chains
 # ---------------------------------------------------------------------------
