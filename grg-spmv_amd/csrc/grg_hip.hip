@@ -1471,28 +1471,112 @@ __global__ __launch_bounds__(kBlock) void row_weighted_finish_kernel(const doubl
 
 constexpr uint32_t kSpinLimit = 1u << 24;  // barrier polls before a wait is declared failed
 
-// Grid-wide barrier of the persistent job kernel. Every wave first waits for
-// its own stores to reach L2 (s_waitcnt 0); one thread per workgroup then
-// writes the XCD's L2 back (agent-scope release), arrives on a monotonic
-// counter, polls it with relaxed loads (a poll must not invalidate caches that
-// working workgroups still use) and, once `target` workgroups have arrived,
-// drops stale L1/L2 lines with one agent-scope acquire. A wait is bounded: past
-// kSpinLimit polls it sets *status and gives up, so the kernel always ends.
-__device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t target, int32_t* status) {
+// Grid barrier of the persistent job kernel, shaped by the caches and by the
+// cost of device-scope atomics. The CUs of one XCD share its L2 (vector L1 is
+// write-through); only agent-scope fences move data between the XCDs' L2s; and
+// device-scope atomics on one word serialize, so no word takes more than a few
+// dozen of them per barrier. Every counter sits in its own slot of kBarStride
+// words (its own 256-byte line) of the barrier buffer, zeroed before each launch:
+constexpr int kBarStride = 64;
+constexpr int kMaxXcd = 16;   // XCC_ID is 4 bits
+constexpr int kMaxSub = 16;   // arrival sub-groups per XCD: workgroup w joins sub-group (w / 8) % 16
+constexpr int kJoinFan = 32;  // registration sub-counters: workgroup w registers on w % 32
+constexpr int kSlotGrid = 0;                              // XCDs done with a barrier (monotonic)
+constexpr int kSlotJoin = 1;                              // registration sub-counters that filled up
+constexpr int kSlotGroups = 2;                            // XCDs holding at least one workgroup
+constexpr int kSlotJoinSub = 3;                           // [kJoinFan]: workgroups registered
+constexpr int kSlotSubs = kSlotJoinSub + kJoinFan;        // [XCD]: its non-empty sub-groups
+constexpr int kSlotXcd = kSlotSubs + kMaxXcd;             // [XCD]: its sub-groups done (monotonic)
+constexpr int kSlotRelease = kSlotXcd + kMaxXcd;          // [XCD]: barriers it is released from
+constexpr int kSlotCount = kSlotRelease + kMaxXcd;        // [XCD][sub-group]: workgroups in it
+constexpr int kSlotSub = kSlotCount + kMaxXcd * kMaxSub;  // [XCD][sub-group]: arrivals (monotonic)
+constexpr int kBarSlots = kSlotSub + kMaxXcd * kMaxSub;
+static_assert(kBarSlots * kBarStride <= GRG_HIP_BARRIER_WORDS, "barrier buffer too small for its slots");
+// s_getreg operand for HW_REG_XCC_ID bits [3:0]: register 20, offset 0, 4 bits.
+constexpr int kHwRegXccId = 20 | (0 << 6) | ((4 - 1) << 11);
+
+// One workgroup's place in the barrier (kept in LDS, used by thread 0).
+struct GridSync {
+  uint32_t* bar;
+  int32_t* status;
+  uint32_t xcd;       // this workgroup's XCD
+  uint32_t sub;       // its arrival sub-group on that XCD
+  uint32_t sub_size;  // workgroups in that sub-group
+  uint32_t subs;      // non-empty sub-groups on that XCD
+  uint32_t groups;    // XCDs holding at least one workgroup of the grid
+};
+
+__device__ __forceinline__ uint32_t* bar_slot(uint32_t* bar, int slot) { return bar + (size_t)slot * kBarStride; }
+
+__device__ __forceinline__ uint32_t atomic_get(const uint32_t* w) {
+  return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t atomic_bump(uint32_t* w) {
+  return __hip_atomic_fetch_add(w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+}
+
+// Poll until *w >= target; past kSpinLimit polls set *status and give up.
+__device__ __forceinline__ void wait_at_least(const uint32_t* w, uint32_t target, int32_t* status) {
+  uint32_t spins = 0;
+  while (atomic_get(w) < target) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > kSpinLimit) {
+      __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+  }
+}
+
+// Kernel start (thread 0): count this workgroup in its sub-group (the first
+// of a sub-group counts it on its XCD, the first of an XCD counts the XCD),
+// register through a two-level counter and wait until the whole grid has
+// registered, so that every count is final. Only atomics are exchanged: no
+// cache maintenance is needed.
+__device__ __forceinline__ GridSync grid_join(uint32_t* bar, int32_t* status) {
+  GridSync g{bar, status, 0u, 0u, 0u, 0u, 0u};
+  g.xcd = (uint32_t)__builtin_amdgcn_s_getreg(kHwRegXccId) & (kMaxXcd - 1);
+  g.sub = (blockIdx.x / 8u) % (uint32_t)kMaxSub;
+  const int cell = (int)(g.xcd * kMaxSub + g.sub);
+  if (atomic_bump(bar_slot(bar, kSlotCount + cell)) == 1u && atomic_bump(bar_slot(bar, kSlotSubs + (int)g.xcd)) == 1u)
+    atomic_bump(bar_slot(bar, kSlotGroups));
+  __builtin_amdgcn_s_waitcnt(0);  // counted before registered
+  const uint32_t j = blockIdx.x % (uint32_t)kJoinFan;
+  const uint32_t in_j = (gridDim.x - j + kJoinFan - 1) / kJoinFan;  // workgroups w < gridDim.x with w % kJoinFan == j
+  if (atomic_bump(bar_slot(bar, kSlotJoinSub + (int)j)) == in_j) atomic_bump(bar_slot(bar, kSlotJoin));
+  wait_at_least(bar_slot(bar, kSlotJoin), min(gridDim.x, (uint32_t)kJoinFan), status);
+  g.sub_size = atomic_get(bar_slot(bar, kSlotCount + cell));
+  g.subs = atomic_get(bar_slot(bar, kSlotSubs + (int)g.xcd));
+  g.groups = atomic_get(bar_slot(bar, kSlotGroups));
+  return g;
+}
+
+// Barrier number n (1, 2, ... within a launch). Every wave first waits for its
+// own stores to reach the XCD's L2. Thread 0 arrives on its sub-group's
+// counter; the last of a sub-group arrives on the XCD's counter; the last of an
+// XCD writes that L2 back once (agent-scope release) and arrives on the grid
+// counter; the last XCD to arrive raises every XCD's release word. Each
+// workgroup waits on its XCD's word and then drops stale L1/L2 lines with an
+// agent-scope acquire (an invalidate at workgroup scope would leave its CU's
+// L1 alone). Every wait is bounded (wait_at_least), so the kernel always ends;
+// a failed wait sets *status.
+__device__ __forceinline__ void grid_barrier(const GridSync& g, uint32_t n) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(2);
-      if (++spins > kSpinLimit) {
-        __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
+    bool last = atomic_bump(bar_slot(g.bar, kSlotSub + (int)(g.xcd * kMaxSub + g.sub))) == g.sub_size * n;
+    if (last) last = atomic_bump(bar_slot(g.bar, kSlotXcd + (int)g.xcd)) == g.subs * n;
+    if (last) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __builtin_amdgcn_s_waitcnt(0);  // the writeback completes before the grid arrival
+      if (atomic_bump(bar_slot(g.bar, kSlotGrid)) == g.groups * n) {
+        for (int x = 0; x < kMaxXcd; ++x)
+          __hip_atomic_store(bar_slot(g.bar, kSlotRelease + x), n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
+    wait_at_least(bar_slot(g.bar, kSlotRelease + (int)g.xcd), n, g.status);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
   }
   __syncthreads();
 }
@@ -1521,12 +1605,14 @@ __device__ __forceinline__ void run_job_block(const GrgHipJob& J, int64_t bx, in
 // the grid; with a barrier counter (persistent launch) the grid meets between
 // phases, without one the launch covers a single phase.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void jobs_kernel(const GrgHipJob* __restrict__ jobs,
+__global__ __launch_bounds__(kBlock, 4) void jobs_kernel(const GrgHipJob* __restrict__ jobs,
                                                       const int32_t* __restrict__ phase_job_off,
                                                       const int64_t* __restrict__ phase_blocks, int32_t phase_begin,
                                                       int32_t phase_end, uint32_t* barrier, int32_t* status) {
   using Acc = typename SumOf<T>::type;
   __shared__ Acc partial[kWavesPerBlock][64];
+  __shared__ GridSync sync;  // in LDS: nothing of the barrier stays live in registers across the gathers
+  if (barrier && threadIdx.x == 0) sync = grid_join(barrier, status);
   for (int32_t p = phase_begin; p < phase_end; ++p) {
     const int32_t j0 = phase_job_off[p], j1 = phase_job_off[p + 1];
     const int64_t nb = phase_blocks[p];
@@ -1542,12 +1628,12 @@ __global__ __launch_bounds__(kBlock) void jobs_kernel(const GrgHipJob* __restric
       run_job_block<T>(J, local % J.gx, (int)(local / J.gx), partial);
       __syncthreads();  // partial[] is reused by the next block
     }
-    if (barrier && p + 1 < phase_end) grid_barrier(barrier, (uint32_t)(p - phase_begin + 1) * gridDim.x, status);
+    if (barrier && p + 1 < phase_end) grid_barrier(sync, (uint32_t)(p - phase_begin + 1));
   }
 }
 
 __global__ void reset_counter_kernel(uint32_t* bar) {
-  if (threadIdx.x == 0) *bar = 0u;
+  for (int slot = (int)threadIdx.x; slot < kBarSlots; slot += (int)blockDim.x) *bar_slot(bar, slot) = 0u;
 }
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
@@ -2032,7 +2118,7 @@ int grg_hip_jobs_run(const GrgHipJob* jobs, const int32_t* phase_job_off, const 
     if (grid < 1 || grid > max_grid)
       return fail("persistent grid " + std::to_string(grid) + " outside [1, " + std::to_string(max_grid) + "]");
     if (nphase == 0) return 0;
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, barrier);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(256), 0, s, barrier);
     if (dtype == GRG_HIP_F64)
       hipLaunchKernelGGL(jobs_kernel<double>, dim3(grid), dim3(kBlock), 0, s, jobs, phase_job_off, phase_blocks, 0, nphase, barrier, status);
     else

@@ -13,7 +13,8 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 15
+ABI_VERSION = 16
+BARRIER_WORDS = 40960  # GRG_HIP_BARRIER_WORDS: uint32 words of the persistent barrier buffer
 STATIC_SCRATCH_DOUBLES = 1 << 22
 F64 = 0
 F32C = 1

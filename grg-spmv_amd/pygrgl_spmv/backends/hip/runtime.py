@@ -464,7 +464,7 @@ class HipBatchedSweep:
             self._job_off_dev = torch.from_numpy(job_off).to(dev)
             self._blocks_dev = torch.from_numpy(blocks.copy() if blocks.size else np.zeros(1, np.int64)).to(dev)
             self._blocks_host = np.ascontiguousarray(blocks if blocks.size else np.zeros(1, np.int64))
-            self._barrier = torch.zeros((1,), dtype=torch.int32, device=dev)
+            self._barrier = torch.zeros((ffi.BARRIER_WORDS,), dtype=torch.int32, device=dev)
             self._status = torch.zeros((1,), dtype=torch.int32, device=dev)
             max_grid = runtime._lib.jobs_max_grid(runtime._dtype_code, int(runtime._root_stream.cuda_stream))
             want = int(self._blocks_host.max()) if self.num_phases else 1
@@ -480,6 +480,71 @@ class HipBatchedSweep:
                 rt._lib.jobs_run(self._jobs_dev.data_ptr(), self._job_off_dev.data_ptr(), self._blocks_dev.data_ptr(),
                                  self._blocks_host, rt._dtype_code, self.num_phases, self.persistent, self.grid,
                                  self._barrier.data_ptr(), self._status.data_ptr(), int(rt._root_stream.cuda_stream))
+
+    def tune_grid(self, candidates=None, *, reps: int = 3) -> dict:
+        """Time the persistent launch at each candidate grid (default: a quarter, half,
+        three quarters and all of the co-resident workgroups) and keep the fastest. A
+        larger grid keeps more gathers in flight per phase; a smaller one makes every
+        barrier cheaper. The batch runs ``reps + 1`` times per candidate and its
+        outputs overwrite the input views (they share storage): set the inputs again
+        afterwards. Returns {grid: best milliseconds}."""
+        torch = _torch()
+        if not self.persistent:
+            raise ValueError("tune_grid applies to a persistent batched sweep")
+        if candidates is None:
+            candidates = sorted({max(1, self.max_grid * q // 4) for q in (1, 2, 3, 4)})
+        rt = self._runtime
+        times = {}
+        for grid in candidates:
+            if not 1 <= int(grid) <= self.max_grid:
+                raise ValueError(f"grid must be in [1, {self.max_grid}]")
+            self.grid = int(grid)
+            best = float("inf")
+            for _ in range(int(reps) + 1):
+                with torch.cuda.device(rt.device):
+                    s = rt._caller_stream   # the batch joins the runtime's caller stream on both sides
+                    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    t0.record(s)
+                    self()
+                    t1.record(s)
+                t1.synchronize()
+                best = min(best, t0.elapsed_time(t1))
+            self.check()
+            times[int(grid)] = best
+        self.grid = min(times, key=times.get)
+        return times
+
+    def barrier_costs(self, *, barriers: int = 64, reps: int = 5) -> dict:
+        """Fixed costs of the persistent kernel at this sweep's grid, in microseconds:
+        ``one_phase_us``, a persistent run of one empty phase (counter reset, launch
+        and registration; no workgroup touches a job), and ``barrier_us``, one grid
+        barrier (runs of ``barriers + 1`` empty phases against runs of one). Each run
+        is timed with HIP events on the launch stream; best of ``reps``."""
+        torch = _torch()
+        rt = self._runtime
+        dev = rt.device
+
+        def best(nphase: int) -> float:
+            off = torch.zeros((nphase + 1,), dtype=torch.int32, device=dev)
+            blocks = torch.zeros((nphase,), dtype=torch.int64, device=dev)
+            host = np.zeros(nphase, dtype=np.int64)
+            times = []
+            with rt._caller_root_scope():
+                s = rt._root_stream
+                for _ in range(reps + 1):
+                    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    t0.record(s)
+                    rt._lib.jobs_run(self._jobs_dev.data_ptr(), off.data_ptr(), blocks.data_ptr(), host, rt._dtype_code,
+                                     nphase, True, self.grid, self._barrier.data_ptr(), self._status.data_ptr(),
+                                     int(s.cuda_stream))
+                    t1.record(s)
+                    t1.synchronize()
+                    times.append(t0.elapsed_time(t1))
+            self.check()
+            return min(times[1:]) * 1e3
+
+        one = best(1)
+        return {"one_phase_us": one, "barrier_us": (best(barriers + 1) - one) / barriers}
 
     def check(self) -> None:
         """Synchronize; raise if any persistent barrier wait ran past its bound."""

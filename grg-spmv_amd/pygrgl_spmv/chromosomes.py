@@ -39,7 +39,7 @@ def mixed_k(i: int) -> int:
 
 
 def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: int = 3, seed: int = 4,
-        autotune: bool = True) -> dict:
+        autotune: bool = True, tune_grid: bool = True) -> dict:
     import torch
 
     from pygrgl_spmv.backends.hip import HipPlanPair, HipRuntime, plan_hip_layout
@@ -129,12 +129,16 @@ def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: i
                                                   ", not captured (streamed blocks)"))
         if capturable:
             # the same 44 sweeps from job tables: all GRGs' level p in one phase
-            tol = 1e-12 if np.dtype(dtype) == np.float64 else 1e-5
+            tol = 1e-12 if np.dtype(dtype) == np.float64 else 5e-7
             for mode, persistent in (("persistent", True), ("grouped", False)):
                 ups = rt.prepare_batched_sweep([dict(grg=g, direction="up", k=mixed_k(i)) for i, g in enumerate(grgs)],
                                                persistent=persistent)
                 dns = rt.prepare_batched_sweep([dict(grg=g, direction="down", k=mixed_k(i)) for i, g in enumerate(grgs)],
                                                persistent=persistent)
+                grid_ms = {}
+                if persistent and tune_grid:
+                    grid_ms = {"up": ups.tune_grid(), "down": dns.tune_grid()}
+                    log(f"persistent grids (ms per sweep): up {grid_ms['up']} -> {ups.grid}, down {grid_ms['down']} -> {dns.grid}")
 
                 def batched():
                     for v, x in zip(ups.inputs, xs):
@@ -162,7 +166,8 @@ def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: i
                     ev[1].record(cap)
                 torch.cuda.synchronize()
                 res = {"phases_up": ups.num_phases, "phases_down": dns.num_phases, "jobs": ups.num_jobs + dns.num_jobs,
-                       "grid": ups.grid if persistent else None, "eager_ms": ev[0].elapsed_time(ev[1]),
+                       "grid": [ups.grid, dns.grid] if persistent else None, "grid_ms": grid_ms,
+                       "eager_ms": ev[0].elapsed_time(ev[1]),
                        "max_rel_err_vs_per_grg_ops": err}
                 eager_b = [o.cpu().clone() for o in (*ups.outputs, *dns.outputs)]
                 graph = torch.cuda.CUDAGraph()
@@ -184,6 +189,10 @@ def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: i
                         raise AssertionError(f"{mode} batched graph replay {rep} differs from its eager run")
                 res["graph_replay_ms"] = min(times)
                 res["graph_edge_updates_per_s"] = edges / (min(times) / 1e3)
+                if persistent:
+                    res.update(ups.barrier_costs())
+                    log(f"persistent kernel at {ups.grid} workgroups: grid barrier {res['barrier_us']:.2f} us, "
+                        f"one empty phase (reset, launch, registration) {res['one_phase_us']:.2f} us")
                 out[f"batched_{mode}"] = res
                 log(f"batched ({mode}): {res['jobs']} jobs in {res['phases_up']} + {res['phases_down']} phases, eager "
                     f"{res['eager_ms']:.2f} ms, graph replay {res['graph_replay_ms']:.2f} ms, rel err vs per-GRG ops {err:.1e}")

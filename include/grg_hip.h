@@ -37,7 +37,11 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 15
+#define GRG_HIP_ABI_VERSION 16
+
+/* uint32 words of the persistent job kernel's barrier buffer (160 KiB: its
+ * counters sit on separate 256-byte lines). */
+#define GRG_HIP_BARRIER_WORDS 40960
 
 enum GrgHipDtype {
   GRG_HIP_F64 = 0,      /* FP64 values, FP64 accumulation */
@@ -307,7 +311,8 @@ int grg_hip_jobs_max_grid(int32_t dtype, void* stream, int32_t* max_workgroups);
  * persistent = 0: one launch per phase, in stream order. persistent = 1: ONE
  * launch of `grid` workgroups (1 <= grid <= grg_hip_jobs_max_grid) that loop
  * over each phase's workgroups and meet at a grid-wide barrier between
- * phases; `barrier` (one uint32 of device memory) is reset by a kernel on the
+ * phases; `barrier` (GRG_HIP_BARRIER_WORDS uint32 of device memory: the grid
+ * and per-XCD counters of the two-tier barrier) is reset by a kernel on the
  * stream before the launch, and `status` (one int32 of device memory, set
  * to 0 by the caller) becomes 1 if a barrier wait ran past its bound (the
  * results are then invalid). Both are capturable into a hipGraph. */

@@ -775,6 +775,47 @@ def test_batched_sweep_of_several_grgs(artifact_dir, art_a, art_b, persistent):
                 _check(out.cpu().numpy(), w)
 
 
+def test_persistent_full_grid_and_barrier_cost(artifact_dir, art_a, art_b):
+    """The persistent launch at every co-resident workgroup (the grid where each XCD
+    holds most members per sub-group): every output against the oracle on three
+    consecutive runs, and the fixed costs of the two-tier barrier stay small (about
+    9 us per barrier and 15 us per launch at 1024 workgroups on MI355X; the bound
+    leaves room for a noisy box but fails the one-release-per-workgroup form, which
+    cost about 85 us)."""
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    arts = [art_a, art_b]
+    oracles = [O.load_artifact(p) for p in arts]
+    layout = build_hip_layout(arts, requirements=full_requirements(max_k_up=16, max_k_down=16))
+    rng = np.random.default_rng(3855)
+    with HipRuntime(layout) as rt:
+        ga, gb = rt.grgs
+        entries = [dict(grg=ga, direction="up", k=9), dict(grg=ga, direction="down", k=16, use_miss=True),
+                   dict(grg=gb, direction="up", k=2), dict(grg=gb, direction="down", k=5)]
+        probe = rt.prepare_batched_sweep(entries, persistent=True)
+        batch = rt.prepare_batched_sweep(entries, persistent=True, grid=probe.max_grid)
+        assert batch.grid == batch.max_grid
+        gs = (oracles[0], oracles[0], oracles[1], oracles[1])
+        for rep in range(3):
+            xs = []
+            for e, view, g in zip(entries, batch.inputs, gs):
+                x = rng.choice(np.array([-1.0, 1.0]), size=(e["k"], _cols(g, e["direction"])))
+                view.copy_(torch.from_numpy(x))
+                xs.append(x)
+            miss = rng.choice(np.array([-1.0, 1.0]), size=(16, oracles[0].num_mutations))
+            batch.miss_inputs[1].copy_(torch.from_numpy(miss))
+            batch()
+            batch.check()
+            for i, (e, out, x, g) in enumerate(zip(entries, batch.outputs, xs, gs)):
+                want = O.matmul(g, x, e["direction"], miss=miss if i == 1 else None)
+                np.testing.assert_array_equal(out.cpu().numpy(), want)
+        costs = batch.barrier_costs()
+        print(f"grid {batch.grid}: {costs}")
+        assert costs["barrier_us"] <= 25.0 and costs["one_phase_us"] <= 100.0
+
+
 def test_tuned_shapes_follow_leading_dimensions(art_a, oracle_a):
     """A batched sweep at k = 4 (private buffers with ld = 4, where 4 columns per lane are
     admissible) is tuned first; an op of the same k on the runtime's arena (ld = max_k = 6,
