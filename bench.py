@@ -82,7 +82,7 @@ def host_cores() -> dict:
 
 
 def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss: bool, calibrate_mib: int = 0,
-            runtime_kwargs: dict | None = None):
+            runtime_kwargs: dict | None = None, graph: bool = False):
     import torch
 
     from pygrgl_spmv.backends.hip import HipPlanPair, HipRuntime, plan_hip_layout
@@ -90,8 +90,10 @@ def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss:
 
     req = RuntimeRequirements(max_k_up=k, max_k_down=k, need_down_miss_input=use_miss, need_up_miss_output=False,
                               need_init_vector=False, need_init_matrix=False, need_init_xtx=use_miss)
+    # the runtime's caller stream (the stream its ops join and a capture records on)
+    caller = torch.cuda.Stream(device=torch.device("cuda", 0)) if graph else 0
     layout = plan_hip_layout(artifacts=[artifact], pair=HipPlanPair.default(), dtype=dtype, requirements=req,
-                             vram_budget_bytes=0, ring_buffer_size=0, device=0, stream=0)
+                             vram_budget_bytes=0, ring_buffer_size=0, device=0, stream=caller)
     t0 = time.perf_counter()
     with HipRuntime(layout, **(runtime_kwargs or {})) as rt:
         (grg,) = rt.grgs
@@ -106,13 +108,14 @@ def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss:
 
         def load_inputs(which):
             # up's input and down's output share the arena (and down's input is up's output):
-            # inputs are restored outside the timed events
-            if which == "up":
-                up.input.copy_(x)
-            else:
-                down.input.copy_(y)
-                if miss is not None:
-                    down.miss_input.copy_(miss)
+            # inputs are restored outside the timed events, on the runtime's caller stream
+            with torch.cuda.stream(rt.stream):
+                if which == "up":
+                    up.input.copy_(x)
+                else:
+                    down.input.copy_(y)
+                    if miss is not None:
+                        down.miss_input.copy_(miss)
 
         for _ in range(warmup):
             load_inputs("up")
@@ -131,12 +134,37 @@ def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss:
         wall = time.perf_counter() - wall0
         t_up = sum(a.elapsed_time(b) for a, b in ev[0::2]) / steps / 1e3
         t_dn = sum(a.elapsed_time(b) for a, b in ev[1::2]) / steps / 1e3
+        t_graph = None
+        if graph:
+            # the same up + down pair captured once into a hipGraph and replayed: one
+            # launch of the whole step (the values are not checked here; the clean pass
+            # below runs eagerly)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=caller):
+                up()
+                down()
+            for _ in range(max(1, warmup)):
+                g.replay()
+            torch.cuda.synchronize()
+            gev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+            with torch.cuda.stream(caller):
+                gev[0].record(caller)
+                for i in range(steps):
+                    g.replay()
+                    gev[i + 1].record(caller)
+            torch.cuda.synchronize()
+            t_graph = min(gev[i].elapsed_time(gev[i + 1]) for i in range(steps)) / 1e3
+            t_graph_mean = gev[0].elapsed_time(gev[-1]) / steps / 1e3
+            log(f"{np.dtype(dtype).name}: captured step replay {t_graph * 1e3:.3f} ms (best), {t_graph_mean * 1e3:.3f} ms (mean)")
+            del g
         # one clean pass for the accuracy report
         load_inputs("up")
         up()
+        torch.cuda.synchronize()
         up_out = up.output.detach().cpu().numpy().astype(np.float64)
         load_inputs("down")
         down()
+        torch.cuda.synchronize()
         down_out = down.output.detach().cpu().numpy().astype(np.float64)
         if calibrate_mib > 0:
             # a plain device copy of known size, for scaling HBM byte counters
@@ -151,7 +179,7 @@ def run_gpu(artifact: Path, *, k: int, dtype, steps: int, warmup: int, use_miss:
         tune = [{"config": key[1], "level": key[2], "k": key[3], **{n: (list(v) if isinstance(v, tuple) else v) for n, v in rec.items()}}
                 for key, rec in rt.tune_log.items()]
     return {"t_up": t_up, "t_down": t_dn, "wall_per_step": wall / steps, "up_out": up_out, "down_out": down_out,
-            "tune": tune}
+            "tune": tune, "t_graph": t_graph}
 
 
 def cpu_baseline(artifact: Path, stats: dict, *, k: int, use_miss: bool, budget_s: float = 20.0):
@@ -289,6 +317,8 @@ def main() -> int:
                     help="write the per-level tuner choices and launch times of each form to this JSON file")
     ap.add_argument("--runtime-option", action="append", default=[], metavar="NAME=VALUE",
                     help="keyword argument of HipRuntime for both value types, e.g. panels=True (repeatable)")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the up + down pair captured into one hipGraph and replayed")
     ap.add_argument("--gather-ceiling", action="store_true",
                     help="instead of the sweep: the sweep kernel's gather rate on synthetic random and windowed indices")
     args = ap.parse_args()
@@ -331,7 +361,7 @@ def main() -> int:
     f64 = None
     if not args.no_fp64:
         f64 = run_gpu(artifact, k=k, dtype=np.float64, steps=args.steps, warmup=args.warmup, use_miss=use_miss,
-                      calibrate_mib=args.calibrate_copy_mib, runtime_kwargs=runtime_kwargs)
+                      calibrate_mib=args.calibrate_copy_mib, runtime_kwargs=runtime_kwargs, graph=args.graph)
         t64 = f64["t_up"] + f64["t_down"]
         b64 = algorithmic_bytes(stats, k, 8, "up", False) + algorithmic_bytes(stats, k, 8, "down", use_miss)
         log(f"fp64: up {f64['t_up']*1e3:.3f} ms, down {f64['t_down']*1e3:.3f} ms, {eu / t64 / 1e9:.2f} G edge-updates/s, "
@@ -339,13 +369,18 @@ def main() -> int:
         variants["fp64"] = {"value": eu / t64, "ms_per_step": t64 * 1e3, "ms_up": f64["t_up"] * 1e3,
                             "ms_down": f64["t_down"] * 1e3, "algorithmic_bytes": b64, "algorithmic_GBs": b64 / t64 / 1e9,
                             "wall_ms_per_step": f64["wall_per_step"] * 1e3}
+        if f64["t_graph"] is not None:
+            variants["fp64"]["graph_ms_per_step"] = f64["t_graph"] * 1e3
     if not args.no_fp32c:
         f32 = run_gpu(artifact, k=k, dtype=np.float32, steps=args.steps, warmup=args.warmup, use_miss=use_miss,
-                      calibrate_mib=args.calibrate_copy_mib if args.no_fp64 else 0, runtime_kwargs=runtime_kwargs)
+                      calibrate_mib=args.calibrate_copy_mib if args.no_fp64 else 0, runtime_kwargs=runtime_kwargs,
+                      graph=args.graph)
         t32 = f32["t_up"] + f32["t_down"]
         b32 = algorithmic_bytes(stats, k, 4, "up", False) + algorithmic_bytes(stats, k, 4, "down", use_miss)
         v = {"value": eu / t32, "ms_per_step": t32 * 1e3, "ms_up": f32["t_up"] * 1e3, "ms_down": f32["t_down"] * 1e3,
              "algorithmic_bytes": b32, "algorithmic_GBs": b32 / t32 / 1e9, "wall_ms_per_step": f32["wall_per_step"] * 1e3}
+        if f32["t_graph"] is not None:
+            v["graph_ms_per_step"] = f32["t_graph"] * 1e3
         msg = f"fp32c: up {f32['t_up']*1e3:.3f} ms, down {f32['t_down']*1e3:.3f} ms, {eu / t32 / 1e9:.2f} G edge-updates/s"
         if f64 is not None:
             def col_err(a, b):

@@ -1000,31 +1000,46 @@ __global__ __launch_bounds__(kBlock) void tile_gather_kernel(const GrgHipGatherA
 }
 
 // ---- panel launch -----------------------------------------------------------
+//
+// A panel is R = panel_rows consecutive rows of one launch (256, 512 or 1024).
+// Its distinct sources, in key order, are cut into passes of at most
+// panel_staged (<= 256) sources and panel_max_entries entries. One workgroup of
+// 1024 threads runs one panel: lane groups of kl lanes own rows_per_group
+// consecutive rows each and keep their sums in registers over all passes; a
+// pass stages its sources in LDS (one kl-lane vector of one column chunk per
+// thread and piece) and every row then sums its entries of the pass from LDS.
+// A source shared by several rows of the panel is read from memory once per
+// panel instead of once per entry: on the sample level of a down sweep, where
+// each sample sums hundreds of parents, 1024-row panels read each parent row
+// about 1.1 times instead of about 9 times.
 
-constexpr int kPanelWords = 1;  // 16-byte words of one pass's entry positions per thread (4 KiB per pass)
+constexpr int kPanelBlock = 1024;                   // threads of a panel workgroup
+constexpr int kPanelEntryBytes = 8 * kPanelBlock;  // one 8-byte word of entry positions per thread and pass
 
 // LDS of a panel workgroup, in bytes: the panel's pass source and entry offsets
-// (int32 each), then two pass buffers, each one pass's staged rows, its row
-// offsets (uint16) and its entry positions (uint8, kPanelWords words per thread).
+// (int32 each), then two pass buffers, each one pass's staged rows, the end
+// offsets of the panel's rows within the pass (uint16) and its entry positions
+// (uint8, one 8-byte word per thread).
 struct PanelLds {
   int64_t meta, staged, rowoff, entries;
   __host__ __device__ int64_t buffer() const { return staged + rowoff + entries; }
   __host__ __device__ int64_t total() const { return meta + 2 * buffer(); }
 };
 
-__host__ __device__ inline PanelLds panel_lds_layout(int64_t max_passes, int64_t staged_rows, int64_t stride_bytes,
-                                                     int64_t panel_rows) {
+constexpr int kPanelPad = 255;  // entry position of padding entries: staged row 255, kept zero
+
+__host__ __device__ inline PanelLds panel_lds_layout(int64_t max_passes, int64_t stride_bytes, int64_t panel_rows) {
   PanelLds t;
   t.meta = round16(2 * (max_passes + 1) * 4);
-  t.staged = round16(staged_rows * stride_bytes);
-  t.rowoff = round16((panel_rows + 1) * 2);
-  t.entries = (int64_t)kPanelWords * 16 * kBlock;
+  t.staged = round16((kPanelPad + 1) * stride_bytes);
+  t.rowoff = round16(panel_rows * 2);
+  t.entries = kPanelEntryBytes;
   return t;
 }
 
-typedef uint32_t PanelWord __attribute__((ext_vector_type(4)));
+typedef uint32_t PanelWord __attribute__((ext_vector_type(2)));
 
-// Keys of pass j's staged pieces (kl-lane vectors): the pass stages sources
+// Keys of one pass's staged pieces (kl-lane vectors): the pass stages sources
 // pan_src[s0 .. s0 + n) (n >= 1), at most panel_staged of them. Every thread
 // loads a key for each of its PPT pieces (absent pieces reload the pass's last
 // key and are marked 0xffffffff), so the load count never depends on the pass.
@@ -1034,21 +1049,25 @@ __device__ __forceinline__ void panel_keys(const GrgHipGatherArgs& a, int32_t s0
   const auto src = global_ptr(a.pan_src);
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
-    const int i = threadIdx.x + u * kBlock;
+    const int i = threadIdx.x + u * kPanelBlock;
     const int r = i < n_pieces ? (i >> kl_shift) : (n > 0 ? n - 1 : 0);
     const uint32_t k = src[s0 + r];
     key[u] = i < n_pieces ? k : 0xffffffffu;
   }
 }
 
-// Loads of one pass into registers: the staged pieces by key (absent pieces and
-// columns past k load row 0, column 0 of the first segment and are zeroed), the
-// pass's entry positions [e0, e1) as 16-byte words, and one row offset. Branch
-// free, so the compiler waits for the oldest pass in flight alone.
+// Loads of one pass into registers: the staged pieces by key, the pass's entry
+// positions [e0, e1) as 8-byte words (one per thread), and the end offset of row
+// threadIdx.x of the panel within the pass. Absent pieces and columns past k load
+// row 0, column 0 of the first segment, words past the pass reload its first
+// word, and threads past the panel's rows reload its last offset: none of them is
+// stored or read, so the loaded registers are never touched before their store
+// and the compiler keeps every pass in flight until then (a select on a loaded
+// value would make it wait for all loads in flight).
 template <typename T, int CPL, int PPT>
 __device__ __forceinline__ void panel_issue(const GrgHipGatherArgs& a, const uint32_t (&key)[PPT], int col0, int kl_shift,
-                                            int32_t e0, int32_t e1, int64_t rowoff_at, T (&v)[PPT][CPL],
-                                            PanelWord (&w)[kPanelWords], uint32_t& roff) {
+                                            int32_t e0, int32_t e1, int64_t rowoff_at, T (&v)[PPT][CPL], PanelWord& w,
+                                            uint32_t& roff) {
   const int KL = 1 << kl_shift;
   const T* src0 = static_cast<const T*>(a.seg[0].src);
   const T* src1 = a.nseg > 1 ? static_cast<const T*>(a.seg[1].src) : src0;
@@ -1058,7 +1077,7 @@ __device__ __forceinline__ void panel_issue(const GrgHipGatherArgs& a, const uin
   const int64_t ld2 = a.nseg > 2 ? a.seg[2].ld : ld0;
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
-    const int i = threadIdx.x + u * kBlock;
+    const int i = threadIdx.x + u * kPanelBlock;
     const int c = col0 + (i & (KL - 1)) * CPL;
     const bool ok = key[u] != 0xffffffffu && c < a.k;
     const uint32_t kk = ok ? key[u] : 0u;
@@ -1066,146 +1085,124 @@ __device__ __forceinline__ void panel_issue(const GrgHipGatherArgs& a, const uin
     const int64_t srow = (int64_t)(kk & 0x3fffffffu);
     const T* base = sg == 0 ? src0 : (sg == 1 ? src1 : src2);
     const int64_t ld = sg == 0 ? ld0 : (sg == 1 ? ld1 : ld2);
-    T tmp[CPL];
-    load_raw_global<T, CPL>(base + srow * ld + (ok ? c : 0), tmp);
-#pragma unroll
-    for (int cc = 0; cc < CPL; ++cc) v[u][cc] = ok ? tmp[cc] : T(0);
+    load_raw_global<T, CPL>(base + srow * ld + (ok ? c : 0), v[u]);
   }
-  const int32_t w0 = e0 >> 4;
-  int32_t nw = ((e1 + 15) >> 4) - w0;
-  if (nw > kPanelWords * kBlock) nw = kPanelWords * kBlock;  // never more than the LDS holds
+  const int32_t w0 = e0 >> 3;
+  int32_t nw = ((e1 + 7) >> 3) - w0;
+  if (nw > kPanelBlock) nw = kPanelBlock;  // never more than the LDS holds
   const auto ent = global_ptr(reinterpret_cast<const PanelWord*>(a.pan_lidx));
-#pragma unroll
-  for (int u = 0; u < kPanelWords; ++u) {
-    const int i = threadIdx.x + u * kBlock;
-    const PanelWord word = ent[w0 + (i < nw ? i : 0)];  // word w0 is readable (16 bytes of padding)
-    w[u] = i < nw ? word : PanelWord{0u, 0u, 0u, 0u};
-  }
+  const int i = threadIdx.x;
+  w = ent[w0 + (i < nw ? i : 0)];  // word w0 is readable (padding after the entries)
+  // pass_rowoff holds R + 1 offsets per pass, the first 0: thread i reads row i's end
   const int R = a.panel_rows;
-  const int tr = (int)threadIdx.x <= R ? (int)threadIdx.x : R;
-  const uint32_t ro = (uint32_t)global_ptr(a.pass_rowoff)[rowoff_at + tr];
-  roff = (int)threadIdx.x <= R ? ro : 0u;
+  roff = (uint32_t)global_ptr(a.pass_rowoff)[rowoff_at + (i < R ? i + 1 : R)];
 }
 
 // One pass from registers into an LDS pass buffer.
 template <typename T, int CPL, int PPT>
 __device__ __forceinline__ void panel_store(int n_pieces, int kl_shift, int stride, int R, const T (&v)[PPT][CPL],
-                                            const PanelWord (&w)[kPanelWords], uint32_t roff, T* buf, uint16_t* rowoff,
-                                            PanelWord* ent) {
+                                            const PanelWord& w, uint32_t roff, T* buf, uint16_t* ends, PanelWord* ent) {
   const int KL = 1 << kl_shift;
 #pragma unroll
   for (int u = 0; u < PPT; ++u) {
-    const int i = threadIdx.x + u * kBlock;
+    const int i = threadIdx.x + u * kPanelBlock;
     if (i < n_pieces) store_raw<T, CPL>(buf + (i >> kl_shift) * stride + (i & (KL - 1)) * CPL, v[u]);
   }
-#pragma unroll
-  for (int u = 0; u < kPanelWords; ++u) ent[threadIdx.x + u * kBlock] = w[u];
-  if ((int)threadIdx.x <= R) rowoff[threadIdx.x] = (uint16_t)roff;
+  ent[threadIdx.x] = w;
+  if ((int)threadIdx.x < R) ends[threadIdx.x] = (uint16_t)roff;
 }
 
-// A lane group's row: its entries of one stored pass, summed from LDS.
-template <typename T, int CPL, typename Acc>
-__device__ __forceinline__ void panel_sum(const T* lds_col, const uint16_t* rowoff, const uint8_t* ent, int gid, int e, int ES,
-                                          int stride, Acc (&acc)[CPL]) {
-  const int p0 = rowoff[gid], p1 = rowoff[gid + 1];
-  for (int jj = p0 + e; jj < p1; jj += 4 * ES) {
-    int li[4];
+// A lane group's rows r0 .. r0 + RPG - 1 of the panel: their entries of one
+// stored pass, summed from LDS in ascending entry order. Each row's entries in a
+// pass are padded to a multiple of four with position kPanelPad (a staged row of
+// zeros), so one aligned 4-byte read gives four positions and the four staged-row
+// reads that follow carry no predicates.
+template <typename T, int CPL, int RPG, typename Acc>
+__device__ __forceinline__ void panel_sum(const T* lds_col, const uint16_t* ends, const uint32_t* ent4, int r0, int stride,
+                                          Acc (&acc)[RPG][CPL]) {
+  int p0 = r0 > 0 ? (int)ends[r0 - 1] : 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int k = jj + u * ES;
-      li[u] = k < p1 ? (int)ent[k] : -1;
+  for (int t = 0; t < RPG; ++t) {
+    const int p1 = ends[r0 + t];
+    for (int j = p0 >> 2; j < (p1 >> 2); ++j) {
+      const uint32_t w = ent4[j];
+      T x[4][CPL];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) load_raw<T, CPL>(lds_col + (int)((w >> (8 * u)) & 255u) * stride, x[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[t][c].add(x[u][c]);
     }
-    T x[4][CPL];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (li[u] >= 0) {
-        load_raw<T, CPL>(lds_col + li[u] * stride, x[u]);
-      } else {
-#pragma unroll
-        for (int c = 0; c < CPL; ++c) x[u][c] = T(0);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int c = 0; c < CPL; ++c) acc[c].add(x[u][c]);
+    p0 = p1;
   }
 }
 
-// Panel launch: workgroup t owns panel t (panel_rows consecutive rows, one per
-// lane group; t in the launch's XCD block order). The panel's pass offsets are
-// read into LDS once. Its passes then stream through a pipeline: D passes are
-// in flight in registers (each thread's pieces of the pass's staged source rows,
-// its words of the pass's entry positions and one row offset) while the pass
-// before them is summed from one of two LDS buffers; the keys of the next pass
-// to issue are loaded one pass ahead. Per pass: store the oldest pass in flight
-// to the buffer the pass before last used, one barrier, issue the pass D ahead
-// into the freed registers (past the panel's last pass, the last pass again:
-// the load count stays the same and those loads are never stored), sum the
-// stored pass. The barrier also orders the buffer's reuse: every thread
-// finished summing from it (two passes earlier) before any thread reaches the
-// barrier of the pass in between. Each lane group keeps its row's sums in
-// registers over all passes and writes the row once. Heavy rows follow as
-// ordinary workgroup rows.
-template <typename T, int CPL, int PPT, int D>
-__global__ __launch_bounds__(kBlock) void panel_gather_kernel(const GrgHipGatherArgs a, int g_shift, int kl_shift) {
+// Panel launch: workgroup t (of 1024 threads) owns panel t (panel_rows
+// consecutive rows; t in the launch's XCD block order). Lane group g (kl lanes,
+// one kl * cpl column chunk) owns rows g * RPG .. g * RPG + RPG - 1 of the panel.
+// The panel's pass offsets are read into LDS once. Its passes then stream
+// through a pipeline: D passes are in flight in registers (each thread's pieces
+// of the pass's staged source rows, its word of the pass's entry positions and
+// one row's end offset) while the pass before them is summed from one of two
+// LDS buffers; the keys of the next pass to issue are loaded one pass ahead. Per
+// pass: store the oldest pass in flight to the buffer the pass before last
+// used, one barrier, issue the pass D ahead into the freed registers (past the
+// panel's last pass, the last pass again: the load count stays the same and
+// those loads are never stored), sum the stored pass. The barrier also orders
+// the buffer's reuse: every thread finished summing from it (two passes
+// earlier) before any thread reaches the barrier of the pass in between. Each
+// lane group keeps its rows' sums in registers over all passes and writes each
+// row once. Heavy rows (no entries in the panels) are produced by a separate
+// launch of the per-row kernel.
+template <typename T, int CPL, int RPG, int PPT, int D>
+__global__ __launch_bounds__(kPanelBlock) void panel_gather_kernel(const GrgHipGatherArgs a, int kl_shift) {
   using Acc = typename SumOf<T>::type;
   extern __shared__ __align__(16) unsigned char panel_lds[];
-  __shared__ Acc partial[kWavesPerBlock][64];
-  const int lane = threadIdx.x & 63;
   const int KL = 1 << kl_shift;
   const int64_t np = a.num_panels;
-  int64_t t = blockIdx.x;
-  if (t >= np) {
-    workgroup_row<T, CPL, 8>(a, a.heavy_rows[t - np], t - np, (int)blockIdx.y, kl_shift, partial);
-    return;
-  }
-  t = xcd_order(t, np, a.flags);
+  const int64_t t = xcd_order((int64_t)blockIdx.x, np, a.flags);
   const int R = a.panel_rows;
   const int chunk = KL * CPL;
   const int stride = chunk + tile_pad<T, CPL>();
   const int col0 = (int)blockIdx.y * chunk;
-  const PanelLds lay = panel_lds_layout(a.panel_max_passes, a.panel_staged, (int64_t)stride * (int64_t)sizeof(T), R);
+  const PanelLds lay = panel_lds_layout(a.panel_max_passes, (int64_t)stride * (int64_t)sizeof(T), R);
   int32_t* msrc = reinterpret_cast<int32_t*>(panel_lds);
   int32_t* ment = msrc + (a.panel_max_passes + 1);
   unsigned char* buf0 = panel_lds + lay.meta;
   unsigned char* buf1 = buf0 + lay.buffer();
 
-  const int G = 1 << g_shift;
-  const int lg = lane & (G - 1);
-  const int cl = lg & (KL - 1);
-  const int e = lg >> kl_shift;
-  const int ES = G >> kl_shift;
-  const int gid = (int)threadIdx.x >> g_shift;
+  const int cl = threadIdx.x & (KL - 1);
+  const int r0 = ((int)threadIdx.x >> kl_shift) * RPG;  // the lane group's first row in the panel
   const int col = col0 + cl * CPL;
-  const int64_t row = a.row_begin + t * R + gid;
-  bool live = row < a.row_end && col < a.k;
-  if (live && a.heavy_threshold >= 0) {
-    int64_t deg = 0;
-    for (int s = 0; s < a.nseg; ++s) {
-      const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
-      deg += ptr[1] - ptr[0];
-    }
-    live = deg <= a.heavy_threshold;  // a heavy row is produced by its heavy workgroup
-  }
-  Acc acc[CPL];
+  const bool col_ok = col < a.k;
+  Acc acc[RPG][CPL];
 #pragma unroll
-  for (int c = 0; c < CPL; ++c) acc[c].zero();
+  for (int r = 0; r < RPG; ++r)
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) acc[r][c].zero();
 
   const int32_t g0 = a.pan_pass[t];
   int npass = a.pan_pass[t + 1] - g0;
   if (npass > a.panel_max_passes) npass = a.panel_max_passes;  // never more than the LDS holds
-  for (int i = threadIdx.x; i <= npass; i += kBlock) {
+  for (int i = threadIdx.x; i <= npass; i += kPanelBlock) {
     msrc[i] = a.pass_src[g0 + i];
     ment[i] = a.pass_ent[g0 + i];
+  }
+  if ((int)threadIdx.x < 2 * stride) {  // staged row kPanelPad of both buffers: zeros
+    T* z = reinterpret_cast<T*>((threadIdx.x < (unsigned)stride ? buf0 : buf1)) + kPanelPad * stride;
+    z[threadIdx.x % stride] = T(0);
   }
   __syncthreads();
 
   if (npass > 0) {  // the same for every thread of the workgroup
-    // pass p's registers: v / w / roff[p % D]; its keys: key[p % D], loaded before the
-    // rows of the pass before it, so waiting for the keys leaves those rows in flight
+    // pass p's registers: v / w / roff[p % D]. Keys run D passes ahead of the rows:
+    // while pass p's rows are in flight, slot p % D of key holds the keys of pass p + D,
+    // loaded right after pass p's rows were issued. Loads complete in issue order
+    // (one vector-memory counter), so each wait is for loads issued D passes earlier
+    // and never for a younger pass.
     T v[D][PPT][CPL];
-    PanelWord w[D][kPanelWords];
+    PanelWord w[D];
     uint32_t roff[D];
     uint32_t key[D][PPT];
     const int last = npass - 1;
@@ -1218,10 +1215,8 @@ __global__ __launch_bounds__(kBlock) void panel_gather_kernel(const GrgHipGather
     for (int j = 0; j < D; ++j) {
       const int p = j < last ? j : last;
       panel_issue<T, CPL, PPT>(a, key[j], col0, kl_shift, ment[p], ment[p + 1], (int64_t)(g0 + p) * (R + 1), v[j], w[j], roff[j]);
-      if (j == 0) {
-        const int pk = D < last ? D : last;  // pass D's keys take pass 0's key slot
-        panel_keys<PPT>(a, msrc[pk], msrc[pk + 1] - msrc[pk], kl_shift, key[0]);
-      }
+      const int pk = j + D < last ? j + D : last;
+      panel_keys<PPT>(a, msrc[pk], msrc[pk + 1] - msrc[pk], kl_shift, key[j]);
     }
     const T* lds_col0 = reinterpret_cast<const T*>(buf0) + cl * CPL;
     const T* lds_col1 = reinterpret_cast<const T*>(buf1) + cl * CPL;
@@ -1236,17 +1231,16 @@ __global__ __launch_bounds__(kBlock) void panel_gather_kernel(const GrgHipGather
                                  roff[j], reinterpret_cast<T*>(b), reinterpret_cast<uint16_t*>(b + lay.staged),
                                  reinterpret_cast<PanelWord*>(b + lay.staged + lay.rowoff));
         __syncthreads();
-        // keys of pass q + D + 1 (slot (j + 1) % D, whose pass q + 1 was issued already), then
-        // the rows of pass q + D (slot j)
-        const int pk = q + D + 1 < last ? q + D + 1 : last;
-        panel_keys<PPT>(a, msrc[pk], msrc[pk + 1] - msrc[pk], kl_shift, key[(j + 1) % D]);
+        // the rows of pass q + D (their keys sit in slot j), then the keys of pass q + 2D
         const int p = q + D < last ? q + D : last;
         panel_issue<T, CPL, PPT>(a, key[j], col0, kl_shift, ment[p], ment[p + 1], (int64_t)(g0 + p) * (R + 1), v[j], w[j],
                                  roff[j]);
-        if (live)
-          panel_sum<T, CPL, Acc>((q & 1) ? lds_col1 : lds_col0, reinterpret_cast<const uint16_t*>(b + lay.staged),
-                                 reinterpret_cast<const uint8_t*>(b + lay.staged + lay.rowoff) + (ment[q] & 15), gid, e, ES,
-                                 stride, acc);
+        const int pk = q + 2 * D < last ? q + 2 * D : last;
+        panel_keys<PPT>(a, msrc[pk], msrc[pk + 1] - msrc[pk], kl_shift, key[j]);
+        if (col_ok)
+          panel_sum<T, CPL, RPG, Acc>((q & 1) ? lds_col1 : lds_col0, reinterpret_cast<const uint16_t*>(b + lay.staged),
+                                      reinterpret_cast<const uint32_t*>(b + lay.staged + lay.rowoff) + ((ment[q] & 7) >> 2), r0,
+                                      stride, acc);
       }
     }
 #pragma unroll
@@ -1259,23 +1253,30 @@ __global__ __launch_bounds__(kBlock) void panel_gather_kernel(const GrgHipGather
                                  roff[j], reinterpret_cast<T*>(b), reinterpret_cast<uint16_t*>(b + lay.staged),
                                  reinterpret_cast<PanelWord*>(b + lay.staged + lay.rowoff));
         __syncthreads();
-        if (live)
-          panel_sum<T, CPL, Acc>((q & 1) ? lds_col1 : lds_col0, reinterpret_cast<const uint16_t*>(b + lay.staged),
-                                 reinterpret_cast<const uint8_t*>(b + lay.staged + lay.rowoff) + (ment[q] & 15), gid, e, ES,
-                                 stride, acc);
+        if (col_ok)
+          panel_sum<T, CPL, RPG, Acc>((q & 1) ? lds_col1 : lds_col0, reinterpret_cast<const uint16_t*>(b + lay.staged),
+                                      reinterpret_cast<const uint32_t*>(b + lay.staged + lay.rowoff) + ((ment[q] & 7) >> 2), r0,
+                                      stride, acc);
       }
     }
   }
-  for (int off = KL; off < G; off <<= 1) {
+  if (!col_ok) return;
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) acc[c].merge_xor(off);
-  }
-  if (live && e == 0) {
-    add_row_terms<T, CPL>(a, row, col, acc);
-    store_row<T, CPL>(a, row, col, acc);
+  for (int r = 0; r < RPG; ++r) {
+    const int64_t row = a.row_begin + t * R + r0 + r;
+    if (row >= a.row_end) break;
+    if (a.heavy_threshold >= 0) {
+      int64_t deg = 0;
+      for (int s = 0; s < a.nseg; ++s) {
+        const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
+        deg += ptr[1] - ptr[0];
+      }
+      if (deg > a.heavy_threshold) continue;  // a heavy row is produced by its heavy workgroup
+    }
+    add_row_terms<T, CPL>(a, row, col, acc[r]);
+    store_row<T, CPL>(a, row, col, acc[r]);
   }
 }
-
 
 // ---- row-group launch (FP64, MFMA) ------------------------------------------
 
@@ -1660,14 +1661,12 @@ int64_t tile_lds_bytes(const GrgHipGatherArgs& a) {
   return tile_lds_layout(a.tile_lds_rows, stride * elem, a.tile_max_rows, a.tile_max_entries).total() + stat;
 }
 
-// Dynamic LDS of a panel launch plus the kernel's static LDS.
+// LDS of a panel launch (all of it dynamic).
 int64_t panel_lds_bytes(const GrgHipGatherArgs& a) {
-  const bool f64 = a.dtype == GRG_HIP_F64;
-  const int64_t elem = f64 ? 8 : 4;
+  const int64_t elem = a.dtype == GRG_HIP_F64 ? 8 : 4;
   const int64_t pad = a.cpl * elem >= 16 ? a.cpl : 16 / elem;
   const int64_t stride = (int64_t)a.kl * a.cpl + pad;
-  const int64_t stat = f64 ? tile_static_lds<double>() : tile_static_lds<float>();
-  return panel_lds_layout(a.panel_max_passes, a.panel_staged, stride * elem, a.panel_rows).total() + stat;
+  return panel_lds_layout(a.panel_max_passes, stride * elem, a.panel_rows).total();
 }
 
 int validate(const GrgHipGatherArgs& a) {
@@ -1678,7 +1677,12 @@ int validate(const GrgHipGatherArgs& a) {
     return fail("cpl must be 1, 2 or 4 (or 8 for FP32 values: 32 bytes per lane)");
   if (!is_pow2(a.kl) || !is_pow2(a.es) || a.kl * a.es > 64) return fail("kl and es must be powers of two with kl * es <= 64");
   if (a.kl * a.cpl > 64) return fail("kl * cpl must be <= 64");
-  {
+  if (a.flags & GRG_HIP_FLAG_PANEL) {
+    const int u = a.unroll ? a.unroll : 8;
+    const int r = a.rows_per_group ? a.rows_per_group : 1;
+    if ((u != 4 && u != 8) || (r != 1 && r != 2 && r != 4))
+      return fail("panel launch: unroll (passes in flight) must be 4 or 8 and rows_per_group 1, 2 or 4");
+  } else {
     const int u = a.unroll ? a.unroll : 8;
     const int r = a.rows_per_group ? a.rows_per_group : 1;
     const bool ok = (r == 1 && (u == 4 || u == 8 || u == 16)) || (r == 2 && (u == 4 || u == 8)) || (r == 4 && u == 4);
@@ -1710,8 +1714,8 @@ int validate(const GrgHipGatherArgs& a) {
   }
   if ((a.flags & (15 << GRG_HIP_XCD_RUN_SHIFT)) && !(a.flags & GRG_HIP_FLAG_XCD_RUNS))
     return fail("a run length without GRG_HIP_FLAG_XCD_RUNS");
-  if ((a.flags & GRG_HIP_FLAG_XCD_RUNS) && (a.flags & (GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_PANEL | GRG_HIP_FLAG_STREAM)))
-    return fail("XCD runs apply to the per-row light path and to tiles (not with XCD_REMAP, PANEL or STREAM)");
+  if ((a.flags & GRG_HIP_FLAG_XCD_RUNS) && (a.flags & (GRG_HIP_FLAG_XCD_REMAP | GRG_HIP_FLAG_STREAM)))
+    return fail("XCD runs apply to the per-row light path, tiles and panels (not with XCD_REMAP or STREAM)");
   if (a.flags & GRG_HIP_FLAG_PANEL) {
     if (a.flags & (GRG_HIP_FLAG_TILED | GRG_HIP_FLAG_PARTIALS | GRG_HIP_FLAG_STREAM | GRG_HIP_FLAG_MERGED))
       return fail("a panel launch is not tiled, merged, streamed or a partials launch");
@@ -1719,13 +1723,15 @@ int validate(const GrgHipGatherArgs& a) {
       return fail("panel launch needs pan_pass, pass_src, pass_ent, pass_rowoff, pan_src and pan_lidx");
     if (a.nseg < 1) return fail("panel launch needs at least one segment");
     if (a.num_panels < 0) return fail("num_panels < 0");
-    if ((int64_t)a.panel_rows * a.kl * a.es != kBlock || a.panel_rows > kBlock / 2)
-      return fail("panel launch: panel_rows * kl * es must be 256, with kl * es >= 2");
+    const int rpg = a.rows_per_group ? a.rows_per_group : 1;
+    if (a.es != 1) return fail("panel launch: es must be 1 (a lane group of kl lanes owns whole rows)");
+    if (rpg > a.kl || (int64_t)a.panel_rows * a.kl != (int64_t)kPanelBlock * rpg)
+      return fail("panel launch: panel_rows must be 1024 / kl * rows_per_group, with rows_per_group <= kl");
     if (a.num_panels * (int64_t)a.panel_rows < a.row_end - a.row_begin) return fail("panel launch: the panels do not cover the rows");
-    if (a.panel_staged < 1 || a.panel_staged > 256 || (int64_t)a.panel_staged * a.kl > 4 * (int64_t)kBlock)
-      return fail("panel launch: panel_staged must be in [1, 256] with panel_staged * kl <= 1024");
+    if (a.panel_staged < 1 || a.panel_staged > kPanelPad || (int64_t)a.panel_staged * a.kl > 2 * (int64_t)kPanelBlock)
+      return fail("panel launch: panel_staged must be in [1, 255] with panel_staged * kl <= 2048");
     if (a.panel_max_passes < 0) return fail("panel launch: panel_max_passes < 0");
-    if (a.panel_max_entries < 0 || (int64_t)a.panel_max_entries + 32 > (int64_t)kPanelWords * kBlock * 16)
+    if (a.panel_max_entries < 0 || (int64_t)a.panel_max_entries + 32 > (int64_t)kPanelEntryBytes)
       return fail("panel launch: panel_max_entries out of range");
     if (reinterpret_cast<uintptr_t>(a.pan_lidx) % 16) return fail("pan_lidx must be 16-byte aligned");
     if (panel_lds_bytes(a) > kTileLdsLimit) return fail("panel launch: one pass exceeds the LDS");
@@ -1808,31 +1814,56 @@ int launch_tiled_typed(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl
   return 0;
 }
 
-template <typename T, int CPL, int PPT, int D>
-int launch_panel_depth(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl_shift, hipStream_t s) {
-  const size_t dyn = (size_t)(panel_lds_bytes(a) - tile_static_lds<T>());
+template <typename T, int CPL, int RPG, int PPT, int D>
+int launch_panel_depth(const GrgHipGatherArgs& a, dim3 grid, int kl_shift, hipStream_t s) {
+  const size_t dyn = (size_t)panel_lds_bytes(a);
   if (dyn > 64 * 1024) {
-    hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_gather_kernel<T, CPL, PPT, D>),
+    hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&panel_gather_kernel<T, CPL, RPG, PPT, D>),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
     if (err != hipSuccess) return fail(std::string("hipFuncSetAttribute: ") + hipGetErrorString(err), 2);
   }
-  hipLaunchKernelGGL((panel_gather_kernel<T, CPL, PPT, D>), grid, dim3(kBlock), dyn, s, a, g_shift, kl_shift);
+  hipLaunchKernelGGL((panel_gather_kernel<T, CPL, RPG, PPT, D>), grid, dim3(kPanelBlock), dyn, s, a, kl_shift);
   return 0;
 }
 
 // passes in flight: 2 for unroll 4, else 3
-template <typename T, int CPL, int PPT>
-int launch_panel_pieces(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl_shift, hipStream_t s) {
-  if (a.unroll == 4) return launch_panel_depth<T, CPL, PPT, 2>(a, grid, g_shift, kl_shift, s);
-  return launch_panel_depth<T, CPL, PPT, 3>(a, grid, g_shift, kl_shift, s);
+template <typename T, int CPL, int RPG, int PPT>
+int launch_panel_pieces(const GrgHipGatherArgs& a, dim3 grid, int kl_shift, hipStream_t s) {
+  if (a.unroll == 4) return launch_panel_depth<T, CPL, RPG, PPT, 2>(a, grid, kl_shift, s);
+  return launch_panel_depth<T, CPL, RPG, PPT, 3>(a, grid, kl_shift, s);
 }
 
+template <typename T, int CPL, int RPG>
+int launch_panel_rows(const GrgHipGatherArgs& a, dim3 grid, int kl_shift, hipStream_t s) {
+  const int64_t pieces = ((int64_t)a.panel_staged * a.kl + kPanelBlock - 1) / kPanelBlock;
+  if (pieces <= 1) return launch_panel_pieces<T, CPL, RPG, 1>(a, grid, kl_shift, s);
+  return launch_panel_pieces<T, CPL, RPG, 2>(a, grid, kl_shift, s);
+}
+
+// The panels of a launch, then its heavy rows (one workgroup each, by the per-row
+// kernel with no light blocks); both write disjoint rows in stream order.
 template <typename T, int CPL>
-int launch_panel_typed(const GrgHipGatherArgs& a, dim3 grid, int g_shift, int kl_shift, hipStream_t s) {
-  const int64_t pieces = ((int64_t)a.panel_staged * a.kl + kBlock - 1) / kBlock;
-  if (pieces <= 1) return launch_panel_pieces<T, CPL, 1>(a, grid, g_shift, kl_shift, s);
-  if (pieces <= 2) return launch_panel_pieces<T, CPL, 2>(a, grid, g_shift, kl_shift, s);
-  return launch_panel_pieces<T, CPL, 4>(a, grid, g_shift, kl_shift, s);
+int launch_panel_typed(const GrgHipGatherArgs& a, int kl_shift, hipStream_t s) {
+  const int chunk = a.kl * a.cpl;
+  const unsigned gy = (unsigned)((a.k + chunk - 1) / chunk);
+  if (a.num_panels > 0) {
+    const dim3 grid((unsigned)a.num_panels, gy);
+    const int rpg = a.rows_per_group ? a.rows_per_group : 1;
+    int rc = 0;
+    if (rpg == 1) rc = launch_panel_rows<T, CPL, 1>(a, grid, kl_shift, s);
+    else if (rpg == 2) rc = launch_panel_rows<T, CPL, 2>(a, grid, kl_shift, s);
+    else rc = launch_panel_rows<T, CPL, 4>(a, grid, kl_shift, s);
+    if (rc) return rc;
+  }
+  if (a.num_heavy > 0) {
+    GrgHipGatherArgs h = a;
+    h.flags = 0;
+    h.es = 1;
+    h.unroll = 8;
+    h.rows_per_group = 1;
+    launch_typed<T, CPL, 8, 1>(h, dim3((unsigned)a.num_heavy, gy), 0, ilog2(a.kl), kl_shift, s);
+  }
+  return 0;
 }
 
 template <typename T, int CPL>
@@ -1911,24 +1942,21 @@ int launch_one(const GrgHipGatherArgs& a, hipStream_t stream) {
     return 0;
   }
   if (a.flags & GRG_HIP_FLAG_PANEL) {
-    gx = a.num_panels + a.num_heavy;
-    if (gx == 0) return 0;
-    if (gx > 0x7fffffffLL) return fail("grid too large");
-    const int chunk = a.kl * a.cpl;
-    dim3 grid((unsigned)gx, (unsigned)((a.k + chunk - 1) / chunk));
+    if (a.num_panels + a.num_heavy == 0) return 0;
+    if (a.num_panels > 0x7fffffffLL || a.num_heavy > 0x7fffffffLL) return fail("grid too large");
     int rc = 0;
     if (a.dtype == GRG_HIP_F64) {
       switch (a.cpl) {
-        case 1: rc = launch_panel_typed<double, 1>(a, grid, g_shift, kl_shift, stream); break;
-        case 2: rc = launch_panel_typed<double, 2>(a, grid, g_shift, kl_shift, stream); break;
-        default: rc = launch_panel_typed<double, 4>(a, grid, g_shift, kl_shift, stream); break;
+        case 1: rc = launch_panel_typed<double, 1>(a, kl_shift, stream); break;
+        case 2: rc = launch_panel_typed<double, 2>(a, kl_shift, stream); break;
+        default: rc = launch_panel_typed<double, 4>(a, kl_shift, stream); break;
       }
     } else {
       switch (a.cpl) {
-        case 1: rc = launch_panel_typed<float, 1>(a, grid, g_shift, kl_shift, stream); break;
-        case 2: rc = launch_panel_typed<float, 2>(a, grid, g_shift, kl_shift, stream); break;
-        case 8: rc = launch_panel_typed<float, 8>(a, grid, g_shift, kl_shift, stream); break;
-        default: rc = launch_panel_typed<float, 4>(a, grid, g_shift, kl_shift, stream); break;
+        case 1: rc = launch_panel_typed<float, 1>(a, kl_shift, stream); break;
+        case 2: rc = launch_panel_typed<float, 2>(a, kl_shift, stream); break;
+        case 8: rc = launch_panel_typed<float, 8>(a, kl_shift, stream); break;
+        default: rc = launch_panel_typed<float, 4>(a, kl_shift, stream); break;
       }
     }
     if (rc) return rc;

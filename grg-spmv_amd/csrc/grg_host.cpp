@@ -210,6 +210,8 @@ void build_tiles(int64_t R, int32_t L, const int64_t* off, int32_t nseg, const i
 // distinct sources (the union of their entries' keys, ascending) are staged
 // max_staged at a time ("passes"). Per pass, every row's entries that fall in
 // the staged window are listed as 8-bit window positions, ascending.
+constexpr int32_t kPanelPad = 255;  // entry position of the padding entries (a zero row in the kernel's LDS)
+
 struct PanelSet {
   int32_t panel_rows = 0;
   int32_t max_staged = 0;
@@ -233,10 +235,13 @@ struct PanelPart {  // panels [p0, p1) built by one thread, offsets local to the
 };
 
 void build_panel_part(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys, int32_t R, int32_t P,
-                      int64_t heavy, int64_t p0, int64_t p1, PanelPart& out) {
+                      int32_t E, int64_t heavy, int64_t p0, int64_t p1, PanelPart& out) {
   std::vector<std::pair<uint32_t, int32_t>> ent;  // (key, row in panel)
   std::vector<uint32_t> uni;
+  std::vector<int32_t> uses;   // entries per distinct source
+  std::vector<int32_t> pass_of, first_of;  // pass of each distinct source; first source of each pass
   std::vector<int32_t> cnt;
+  std::vector<int32_t> rowcnt, touched;  // entries per row of the pass being built; rows with entries
   for (int64_t p = p0; p < p1; ++p) {
     const int64_t r0 = row_begin + p * R, r1 = std::min<int64_t>(row_end, r0 + R);
     ent.clear();
@@ -248,51 +253,96 @@ void build_panel_part(int64_t row_begin, int64_t row_end, const int32_t* eptr, c
     }
     std::sort(ent.begin(), ent.end());
     uni.clear();
+    uses.clear();
     for (size_t i = 0; i < ent.size(); ++i) {
       if (i > 0 && ent[i].first == ent[i - 1].first && ent[i].second == ent[i - 1].second)
         throw std::invalid_argument("a row lists the same source twice");
-      if (uni.empty() || uni.back() != ent[i].first) uni.push_back(ent[i].first);
+      if (uni.empty() || uni.back() != ent[i].first) {
+        uni.push_back(ent[i].first);
+        uses.push_back(0);
+      }
+      ++uses.back();
     }
-    const int32_t np = (int32_t)((uni.size() + P - 1) / P);
+    // passes: runs of consecutive distinct sources, closed before the (P + 1)-th source or
+    // before the padded entries would exceed E. A row's entries in a pass are padded to
+    // a multiple of 4 (with position kPanelPad), so adding a source costs 4 padded
+    // entries for each of its rows whose count in the pass is a multiple of 4 (a
+    // source's padded entries, at most 4 R <= E, always fit an empty pass).
+    pass_of.assign(uni.size(), 0);
+    first_of.clear();
+    {
+      rowcnt.assign((size_t)R, 0);
+      touched.clear();
+      int32_t q = -1, n_src = P;
+      int64_t padded = 0;
+      size_t i0 = 0;
+      for (size_t u = 0; u < uni.size(); ++u) {
+        const size_t i1 = i0 + (size_t)uses[u];  // entries of source u are ent[i0, i1)
+        int64_t delta = 0;
+        for (size_t i = i0; i < i1; ++i) delta += rowcnt[(size_t)ent[i].second] % 4 == 0 ? 4 : 0;
+        if (n_src == P || padded + delta > E) {
+          ++q;
+          first_of.push_back((int32_t)u);
+          n_src = 0;
+          padded = 0;
+          for (int32_t r : touched) rowcnt[(size_t)r] = 0;
+          touched.clear();
+          delta = 4 * (int64_t)uses[u];
+        }
+        pass_of[u] = q;
+        ++n_src;
+        padded += delta;
+        for (size_t i = i0; i < i1; ++i) {
+          const int32_t r = ent[i].second;
+          if (rowcnt[(size_t)r]++ == 0) touched.push_back(r);
+        }
+        i0 = i1;
+      }
+    }
+    const int32_t np = (int32_t)first_of.size();
+    first_of.push_back((int32_t)uni.size());
     out.n_pass.push_back(np);
     out.src_list.insert(out.src_list.end(), uni.begin(), uni.end());
-    // entries bucketed by (pass, row); within a bucket in ascending position
+    // entries bucketed by (pass, row); within a bucket in ascending position, padded to a
+    // multiple of 4 with kPanelPad
     cnt.assign((size_t)np * R, 0);
     size_t u = 0;
     std::vector<int32_t> pos(ent.size());
     for (size_t i = 0; i < ent.size(); ++i) {
       while (uni[u] != ent[i].first) ++u;
       pos[i] = (int32_t)u;
-      ++cnt[(size_t)(u / P) * R + ent[i].second];
+      ++cnt[(size_t)pass_of[u] * R + ent[i].second];
     }
     const size_t base = out.lidx.size();
     std::vector<int64_t> fill((size_t)np * R);
     int64_t acc = 0;
     for (int32_t q = 0; q < np; ++q) {
-      out.pass_src.push_back((int32_t)std::min<int64_t>(P, (int64_t)uni.size() - (int64_t)q * P));
+      out.pass_src.push_back(first_of[q + 1] - first_of[q]);
       const int64_t q0 = acc;
       for (int32_t r = 0; r < R; ++r) {
         out.pass_rowoff.push_back((uint16_t)(acc - q0));
         fill[(size_t)q * R + r] = acc;
-        acc += cnt[(size_t)q * R + r];
+        acc += (cnt[(size_t)q * R + r] + 3) / 4 * 4;
       }
       out.pass_rowoff.push_back((uint16_t)(acc - q0));
-      if (acc - q0 > 65535) throw std::invalid_argument("a pass holds more than 65535 entries");
+      if (acc - q0 > E) throw std::logic_error("a pass holds more entries than its limit");
       out.pass_ent.push_back((int32_t)(acc - q0));
       out.max_pass_entries = std::max<int32_t>(out.max_pass_entries, (int32_t)(acc - q0));
     }
-    out.lidx.resize(base + (size_t)acc);
+    out.lidx.resize(base + (size_t)acc, (uint8_t)kPanelPad);
     for (size_t i = 0; i < ent.size(); ++i) {
-      const int32_t q = pos[i] / P;
-      out.lidx[base + (size_t)fill[(size_t)q * R + ent[i].second]++] = (uint8_t)(pos[i] - q * P);
+      const int32_t q = pass_of[(size_t)pos[i]];
+      out.lidx[base + (size_t)fill[(size_t)q * R + ent[i].second]++] = (uint8_t)(pos[i] - first_of[q]);
     }
   }
 }
 
 PanelSet* build_panels(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys, int32_t R, int32_t P,
-                       int64_t heavy, int32_t threads) {
+                       int32_t E, int64_t heavy, int32_t threads) {
   if (row_end < row_begin || row_begin < 0) throw std::invalid_argument("bad row range");
-  if (R < 1 || R > 256 || P < 1 || P > 256) throw std::invalid_argument("panel_rows and max_staged must be in [1, 256]");
+  if (R < 1 || R > 1024 || P < 1 || P > kPanelPad)
+    throw std::invalid_argument("panel_rows must be in [1, 1024] and max_staged in [1, 255]");
+  if (E < 4 * R || E > 65535) throw std::invalid_argument("max_pass_entries must be in [4 * panel_rows, 65535]");
   const int64_t n_panels = (row_end - row_begin + R - 1) / R;
   const int64_t T = std::max<int64_t>(1, std::min<int64_t>(threads > 0 ? threads : 1, n_panels));
   std::vector<PanelPart> parts((size_t)T);
@@ -301,7 +351,8 @@ PanelSet* build_panels(int64_t row_begin, int64_t row_end, const int32_t* eptr, 
   for (int64_t t = 0; t < T; ++t) {
     pool.emplace_back([&, t]() {
       try {
-        build_panel_part(row_begin, row_end, eptr, keys, R, P, heavy, n_panels * t / T, n_panels * (t + 1) / T, parts[(size_t)t]);
+        build_panel_part(row_begin, row_end, eptr, keys, R, P, E, heavy, n_panels * t / T, n_panels * (t + 1) / T,
+                         parts[(size_t)t]);
       } catch (const std::exception& e) {
         errs[(size_t)t] = e.what();
       }
@@ -412,13 +463,13 @@ const char* grg_host_last_error(void) { return g_last_error.c_str(); }
 extern "C" {
 
 void* grg_host_build_panels(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys, int32_t panel_rows,
-                            int32_t max_staged, int64_t heavy_threshold, int32_t threads) {
+                            int32_t max_staged, int32_t max_pass_entries, int64_t heavy_threshold, int32_t threads) {
   if (!eptr || (!keys && row_end > row_begin)) {
     g_last_error = "null argument";
     return nullptr;
   }
   try {
-    return build_panels(row_begin, row_end, eptr, keys, panel_rows, max_staged, heavy_threshold, threads);
+    return build_panels(row_begin, row_end, eptr, keys, panel_rows, max_staged, max_pass_entries, heavy_threshold, threads);
   } catch (const std::exception& e) {
     g_last_error = e.what();
     return nullptr;

@@ -13,7 +13,7 @@ from ctypes import c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parents[2] / "_lib" / "libgrg_hip.so"
-ABI_VERSION = 16
+ABI_VERSION = 17
 BARRIER_WORDS = 40960  # GRG_HIP_BARRIER_WORDS: uint32 words of the persistent barrier buffer
 STATIC_SCRATCH_DOUBLES = 1 << 22
 F64 = 0
@@ -117,6 +117,12 @@ class HipLibrary:
             raise RuntimeError(
                 f"HIP extension {path} is not built; run __graft_entry__.build() -- the HIP backend has no CPU fallback"
             )
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7): load it
+        # first, so that this library binds to it and torch's streams, events and device
+        # memory are the ones the kernels see (loaded the other way round, the system
+        # runtime would shadow torch's and torch would find no GPU)
+        import torch  # noqa: F401
+
         lib = ctypes.CDLL(str(path))
         lib.grg_hip_gather.argtypes = [ctypes.POINTER(GrgHipGatherArgs), c_void_p]
         lib.grg_hip_gather.restype = ctypes.c_int

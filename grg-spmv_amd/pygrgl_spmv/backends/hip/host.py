@@ -34,7 +34,7 @@ def _lib():
         lib.grg_host_build_tiles.restype = ctypes.c_int
         lib.grg_host_merged_keys.argtypes = [c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]
         lib.grg_host_merged_keys.restype = ctypes.c_int
-        lib.grg_host_build_panels.argtypes = [c_int64, c_int64, c_void_p, c_void_p, c_int32, c_int32, c_int64, c_int32]
+        lib.grg_host_build_panels.argtypes = [c_int64, c_int64, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int64, c_int32]
         lib.grg_host_build_panels.restype = c_void_p
         lib.grg_host_panels_sizes.argtypes = [c_void_p, c_void_p]
         lib.grg_host_panels_sizes.restype = ctypes.c_int
@@ -208,9 +208,10 @@ class Panels:
 PANEL_PAD = 16
 
 
-def build_panels(eptr: np.ndarray, keys: np.ndarray, row_begin: int, row_end: int, *, panel_rows: int = 64,
-                 max_staged: int = 128, heavy_threshold: int = -1, threads: int = 0) -> Panels:
-    """Panels for rows [row_begin, row_end) of entries ``keys[eptr[r]:eptr[r + 1]]``."""
+def build_panels(eptr: np.ndarray, keys: np.ndarray, row_begin: int, row_end: int, *, panel_rows: int = 1024,
+                 max_staged: int = 255, max_pass_entries: int = 8000, heavy_threshold: int = -1, threads: int = 0) -> Panels:
+    """Panels for rows [row_begin, row_end) of entries ``keys[eptr[r]:eptr[r + 1]]``: passes
+    of at most ``max_staged`` distinct sources and ``max_pass_entries`` entries."""
     import os
 
     lib = _lib()
@@ -220,7 +221,7 @@ def build_panels(eptr: np.ndarray, keys: np.ndarray, row_begin: int, row_end: in
         raise ValueError("row range outside eptr")
     n = int(threads) if threads > 0 else min(16, os.cpu_count() or 1)
     h = lib.grg_host_build_panels(int(row_begin), int(row_end), eptr.ctypes.data, keys.ctypes.data, int(panel_rows),
-                                  int(max_staged), int(heavy_threshold), n)
+                                  int(max_staged), int(max_pass_entries), int(heavy_threshold), n)
     if not h:
         raise RuntimeError(f"grg_host_build_panels failed: {lib.grg_host_last_error().decode()}")
     try:

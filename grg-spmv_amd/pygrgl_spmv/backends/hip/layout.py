@@ -252,8 +252,9 @@ def schedule_capacity_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeReq
     return int(configs * per_config)
 
 
-PANEL_ROWS = 64       # rows of one panel of the panel launches
-PANEL_STAGED = 128    # source rows staged per panel pass
+PANEL_ROWS = 1024         # rows of one panel of the panel launches
+PANEL_STAGED = 255        # source rows staged per panel pass (8-bit positions)
+PANEL_PASS_ENTRIES = 8000  # entries of one panel pass (its positions sit in 8 KiB of LDS)
 GROUP_ROWS = 16       # rows of one group of the row-group launches (FP64 values)
 
 
@@ -280,11 +281,11 @@ def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequiremen
     tiles = sum(6 * e + (4 * e if keyed else 0) + 16 + 4 * (N + 1) + 8 * (N + L + 1) for e, keyed in entries)
     # panels (no more than one per PANEL_ROWS rows plus one per level): a uint8 entry
     # position and at most one uint32 staged source per entry, and per pass (at most
-    # one per PANEL_STAGED sources plus one per panel) two int32 offsets and
-    # PANEL_ROWS + 1 uint16 row offsets
+    # one per PANEL_STAGED sources, one per PANEL_PASS_ENTRIES entries and two per
+    # panel) two int32 offsets and PANEL_ROWS + 1 uint16 row offsets
     pan = 0
     for e, _ in entries:
-        passes = e // PANEL_STAGED + panels
+        passes = e // PANEL_STAGED + e // PANEL_PASS_ENTRIES + 2 * panels
         pan += 5 * e + 80 + 4 * (panels + 1) + (8 + 2 * (PANEL_ROWS + 1)) * (passes + 1)
     # row groups (no more than one per GROUP_ROWS rows plus one per level): at most one
     # uint32 key and uint16 mask per entry plus three padding entries per group, an

@@ -43,7 +43,8 @@ import scipy.sparse as sp
 
 from pygrgl_spmv.backends.base import copy_struct_checked, relink_stream_dependencies
 from pygrgl_spmv.backends.hip import ffi, tunecache
-from pygrgl_spmv.backends.hip.layout import PANEL_ROWS, PANEL_STAGED, SPLIT_CHUNK, SPLIT_MIN, HipLayout, enabled_max_k, side_io_rows
+from pygrgl_spmv.backends.hip.layout import (PANEL_PASS_ENTRIES, PANEL_ROWS, PANEL_STAGED, SPLIT_CHUNK, SPLIT_MIN, HipLayout,
+                                             enabled_max_k, side_io_rows)
 from pygrgl_spmv.backends.types import Direction, InitMode
 from pygrgl_spmv.grg import BoundGRG, _CudaMatmulSpec
 from pygrgl_spmv.grg.artifact import _load_grg_spmv_host, iter_artifact_blocks
@@ -72,9 +73,10 @@ KERNEL_UNROLL = 8  # gathers in flight per lane and round in the kernel
 TILE_MAX_STAGED = 128   # staged source rows of one tile (LDS rows)
 TILE_MAX_ROWS = 256     # rows of one tile
 TILE_MAX_ENTRIES = 2048  # gather entries of one tile
-PANEL_MIN_MEAN = 32.0   # levels whose light rows average fewer entries get no panels
-PANEL_MIN_ROWS = 1024   # ... nor levels with fewer rows
-PANEL_MAX_PASS_BYTES = 4096   # entry positions of one pass (with 32 bytes of slack) the panel kernel stages
+PANEL_MIN_MEAN = 3.0    # levels whose light rows average fewer entries get no panels
+PANEL_MIN_ROWS = 4096   # ... nor levels with fewer rows
+PANEL_BLOCK = 1024            # threads of a panel workgroup
+PANEL_MAX_PASS_BYTES = 8 * PANEL_BLOCK   # entry positions of one pass (with 32 bytes of slack) the panel kernel stages
 PANEL_MAX_PASSES = 2048       # passes of one panel (their entry offsets sit in LDS)
 GROUP_MIN_MEAN = 3.0    # levels whose light rows average fewer entries get no row groups
 GROUP_MIN_ROWS = 256    # ... nor levels with fewer rows
@@ -139,17 +141,29 @@ def tile_lds_bytes(dtype_code: int, cpl: int, kl: int, staged_rows: int, max_row
 def panel_lds_bytes(dtype_code: int, cpl: int, kl: int, staged_rows: int, panel_rows: int, max_entries: int,
                     max_passes: int) -> int:
     """LDS of one panel workgroup: the panel's int32 pass source and entry offsets, then
-    two pass buffers, each one pass's staged rows (as in a tile), its uint16 row offsets
-    and PANEL_MAX_PASS_BYTES of entry positions (each part rounded to 16 bytes), and the
-    2 KiB reduction buffer. (``max_entries`` is bounded by PANEL_MAX_PASS_BYTES.)"""
+    two pass buffers, each 256 staged rows (as in a tile; the last one zero, the row of
+    the padding entries), the uint16 end offsets of the panel's rows and
+    PANEL_MAX_PASS_BYTES of entry positions (each part rounded to 16 bytes).
+    (``staged_rows`` is at most 255 and ``max_entries`` bounded by PANEL_MAX_PASS_BYTES.)"""
     elem = 8 if dtype_code == ffi.F64 else 4
     pad = cpl if cpl * elem >= 16 else 16 // elem
 
     def r16(b):
         return (int(b) + 15) // 16 * 16
 
-    buffer = r16(int(staged_rows) * (kl * cpl + pad) * elem) + r16((int(panel_rows) + 1) * 2) + PANEL_MAX_PASS_BYTES
-    return r16(2 * (int(max_passes) + 1) * 4) + 2 * buffer + 8 * 4 * 64
+    buffer = r16(256 * (kl * cpl + pad) * elem) + r16(int(panel_rows) * 2) + PANEL_MAX_PASS_BYTES
+    return r16(2 * (int(max_passes) + 1) * 4) + 2 * buffer
+
+
+def panel_rows_per_group(panel_rows: int, kl: int) -> int:
+    """Rows per lane group of a panel launch with lane groups of ``kl`` lanes (1024 / kl
+    groups per workgroup), or 0 if ``panel_rows`` cannot be split that way (1, 2 or 4
+    rows per group, no more than kl)."""
+    groups = PANEL_BLOCK // int(kl)
+    if int(panel_rows) % groups:
+        return 0
+    rpg = int(panel_rows) // groups
+    return rpg if rpg in (1, 2, 4) and rpg <= int(kl) else 0
 
 
 @dataclass(frozen=True)
@@ -561,7 +575,9 @@ class HipRuntime:
                  tile_max_staged: int = TILE_MAX_STAGED, tile_max_rows: int = TILE_MAX_ROWS,
                  tile_max_entries: int = TILE_MAX_ENTRIES, degree_window: int = DEGREE_WINDOW,
                  degree_min_mean: float = DEGREE_MIN_MEAN, panels: bool = False, panel_min_mean: float = PANEL_MIN_MEAN,
-                 panel_min_rows: int = PANEL_MIN_ROWS, groups: bool = True, group_min_mean: float = GROUP_MIN_MEAN,
+                 panel_min_rows: int = PANEL_MIN_ROWS, panel_rows: int = PANEL_ROWS, panel_staged: int = PANEL_STAGED,
+                 groups: bool = True,
+                 group_min_mean: float = GROUP_MIN_MEAN,
                  group_min_rows: int = GROUP_MIN_ROWS, group_max_fill: float = GROUP_MAX_FILL,
                  tune_cache="default") -> None:
         """``autotune``: time the admissible launch shapes of every level launch when an
@@ -579,11 +595,11 @@ class HipRuntime:
         device node order sorts each run of ``degree_window`` consecutive nodes of a
         level (levels whose mean gather degree in the up sweep is at least
         ``degree_min_mean``) by degree, so the rows a wave sums in lockstep have
-        similar lengths (0 keeps the plain locality order). ``panels`` (off by default:
-        on MI355X the per-row forms are faster at every level measured so far): with the
-        tile tables, also build the panels of the panel launches (levels whose light rows
-        average at least ``panel_min_mean`` entries, with at least ``panel_min_rows``
-        rows) and let the tuner choose them. ``groups`` (FP64 values, with the tile tables):
+        similar lengths (0 keeps the plain locality order). ``panels``: with the tile
+        tables, also build the panels of the panel launches (``panel_rows`` rows each: 256,
+        512 or 1024; levels whose light rows average at least ``panel_min_mean`` entries,
+        with at least ``panel_min_rows`` rows; passes of at most ``panel_staged`` sources)
+        and let the tuner choose them. ``groups`` (FP64 values, with the tile tables):
         build the row groups of the row-group launches (levels with at least ``group_min_rows``
         rows whose light rows average at least ``group_min_mean`` entries, kept when a level's
         group entries are at most ``group_max_fill`` of its entries) and let the tuner choose
@@ -621,6 +637,12 @@ class HipRuntime:
         self.group_max_fill = float(group_max_fill)
         self.panel_min_mean = float(panel_min_mean)
         self.panel_min_rows = int(panel_min_rows)
+        if int(panel_rows) not in (256, 512, 1024):
+            raise ValueError("panel_rows must be 256, 512 or 1024")
+        self.panel_rows = int(panel_rows)
+        if not 1 <= int(panel_staged) <= PANEL_STAGED:
+            raise ValueError(f"panel_staged must be in [1, {PANEL_STAGED}]")
+        self.panel_staged = int(panel_staged)
         self.degree_window = int(degree_window)
         self.degree_min_mean = float(degree_min_mean)
         self._dtype_code = ffi.F64 if np.dtype(layout.dtype) == np.float64 else ffi.F32C
@@ -1037,7 +1059,7 @@ class HipRuntime:
                 "split": [self.split_min, self.split_chunk], "heavy": int(self.layout.heavy_threshold),
                 "tiles": [bool(self.layout.tiles and self.use_tiles), self.tile_max_staged, self.tile_max_rows,
                           self.tile_max_entries],
-                "panels": [self.use_panels, self.panel_min_mean, self.panel_min_rows],
+                "panels": [self.use_panels, self.panel_min_mean, self.panel_min_rows, self.panel_rows, self.panel_staged],
                 "order": [self.degree_window, self.degree_min_mean],
                 "resident": hashlib.blake2b(resident.encode(), digest_size=8).hexdigest()}
 
@@ -1058,6 +1080,8 @@ class HipRuntime:
             return False
         if fl & ffi.FLAG_MERGED and not a.ekey:
             return False
+        if fl & ffi.FLAG_PANEL:
+            return es == 1 and unroll in (4, 8) and rows == panel_rows_per_group(a.panel_rows, kl) != 0
         return (unroll, rows) in ((4, 1), (8, 1), (16, 1), (4, 2), (8, 2), (4, 4))
 
     def _elem(self) -> int:
@@ -1074,8 +1098,8 @@ class HipRuntime:
             sched = art.sched.get((cfg, l))
             if sched is None or hi - lo < max(1, self.panel_min_rows) or sched.mean_degree < self.panel_min_mean:
                 continue
-            pn = build_panels(eptr, keys, lo, hi, panel_rows=PANEL_ROWS, max_staged=PANEL_STAGED,
-                              heavy_threshold=int(sched.heavy_threshold))
+            pn = build_panels(eptr, keys, lo, hi, panel_rows=self.panel_rows, max_staged=self.panel_staged,
+                              max_pass_entries=PANEL_PASS_ENTRIES, heavy_threshold=int(sched.heavy_threshold))
             if pn.max_pass_entries + 32 > PANEL_MAX_PASS_BYTES or pn.max_passes > PANEL_MAX_PASSES:
                 continue
             tag = f"{cfg}_pan{l}"
@@ -1132,7 +1156,7 @@ class HipRuntime:
         return {"pan_pass": D[f"{tag}_pass"].data_ptr(), "pass_src": D[f"{tag}_src_off"].data_ptr(),
                 "pass_ent": D[f"{tag}_ent_off"].data_ptr(), "pass_rowoff": D[f"{tag}_rowoff"].data_ptr(),
                 "pan_src": D[f"{tag}_src"].data_ptr(), "pan_lidx": D[f"{tag}_lidx"].data_ptr(), "num_panels": int(meta[0]),
-                "panel_rows": PANEL_ROWS, "panel_staged": PANEL_STAGED, "panel_max_entries": int(meta[1]),
+                "panel_rows": self.panel_rows, "panel_staged": self.panel_staged, "panel_max_entries": int(meta[1]),
                 "panel_max_passes": int(meta[2])}
 
     def _tile_fields(self, art: _HipArtifact, cfg: str, level: int):
@@ -1202,14 +1226,20 @@ class HipRuntime:
                 a.unroll = 4 if a.unroll == 4 else 8
                 a.rows_per_group = 1
         if a.flags & ffi.FLAG_PANEL:
-            # a panel launch needs the level's panels and one panel row per lane group
-            group = 256 // PANEL_ROWS
+            # a panel launch needs the level's panels and lane groups that own 1, 2 or 4 rows
             if not panels:
                 a.flags &= ~ffi.FLAG_PANEL
             else:
-                a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_MERGED)
-                a.kl = min(a.kl, group)
-                a.es = group // a.kl
+                a.flags &= ~(ffi.FLAG_TILED | ffi.FLAG_STREAM | ffi.FLAG_MERGED | ffi.FLAG_GROUP_KEYS)
+                while a.kl > 1 and not panel_rows_per_group(a.panel_rows, a.kl):
+                    a.kl //= 2
+                a.es = 1
+                a.rows_per_group = panel_rows_per_group(a.panel_rows, a.kl)
+                if not a.rows_per_group:
+                    a.flags &= ~ffi.FLAG_PANEL
+                    a.rows_per_group = 1
+                elif a.unroll not in (4, 8):
+                    a.unroll = 8
         if a.flags & ffi.FLAG_GROUPS:
             # a row-group launch needs the level's groups (FP64) and one 16-column chunk
             if not groups or a.dtype != ffi.F64 or a.cpl not in (1, 2, 4):
@@ -1225,8 +1255,8 @@ class HipRuntime:
             else:
                 while a.kl > 1 and a.kl * a.cpl * self._elem() > ffi.TILE_CHUNK_BYTES:
                     a.kl //= 2
-        if a.flags & (ffi.FLAG_XCD_REMAP | ffi.FLAG_PANEL | ffi.FLAG_STREAM):
-            a.flags &= ~(ffi.FLAG_XCD_RUNS | XCD_RUN_BITS)  # runs apply to the light path and to tiles
+        if a.flags & (ffi.FLAG_XCD_REMAP | ffi.FLAG_STREAM):
+            a.flags &= ~(ffi.FLAG_XCD_RUNS | XCD_RUN_BITS)  # runs apply to the light path, tiles and panels
         if a.kl * a.es > 64 and not a.flags & (ffi.FLAG_PANEL | ffi.FLAG_GROUPS):
             a.es = 64 // a.kl
         if (a.flags & ffi.FLAG_GROUP_KEYS) and ((not a.flags & ffi.FLAG_MERGED and a.nseg != 1) or a.rows_per_group > 1
@@ -1340,18 +1370,22 @@ class HipRuntime:
                         for u in (4, 8):
                             time_shape(cpl, kl, es_s, u, 1, base)
                 if a.pan_pass:
-                    # the panel form: one panel row per lane group (kl * es = 256 / panel_rows)
-                    group = 256 // a.panel_rows
+                    # the panel form: lane groups of kl lanes owning panel_rows * kl / 1024 rows each
                     n_before = len(timings)
                     for cpl, kl in column_shapes(a.dtype, a.k, lds):
-                        for klp in sorted({min(kl, group), max(1, min(kl, group) // 2)}):
-                            if a.panel_staged * klp > 1024 or panel_lds_bytes(a.dtype, cpl, klp, a.panel_staged, a.panel_rows, a.panel_max_entries, a.panel_max_passes) > TILE_LDS_LIMIT:
+                        for klp in sorted({kl, max(1, kl // 2)}):
+                            rpg = panel_rows_per_group(a.panel_rows, klp)
+                            if not rpg or a.panel_staged * klp > 2 * PANEL_BLOCK or panel_lds_bytes(
+                                    a.dtype, cpl, klp, a.panel_staged, a.panel_rows, a.panel_max_entries,
+                                    a.panel_max_passes) > TILE_LDS_LIMIT:
                                 continue
                             for u in (4, 8):  # two or three passes in flight
-                                time_shape(cpl, klp, group // klp, u, 1, ffi.FLAG_PANEL)
+                                time_shape(cpl, klp, 1, u, rpg, ffi.FLAG_PANEL)
                     if len(timings) > n_before:
                         _, cpl, kl, es, unroll, rows, _ = min(timings[n_before:])
                         time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_PANEL | ffi.FLAG_XCD_REMAP)
+                        for log_run in XCD_RUN_LOGS:
+                            time_shape(cpl, kl, es, unroll, rows, ffi.FLAG_PANEL | xcd_runs_flags(log_run))
                 if a.tile_row:
                     # the LDS-staged form of the same launch: column chunks of at most
                     # TILE_CHUNK_BYTES, every lane-group width

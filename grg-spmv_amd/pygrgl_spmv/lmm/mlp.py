@@ -22,6 +22,8 @@ class MlpLibrary:
     def __init__(self, path: Path = LIB_PATH):
         if not Path(path).exists():
             raise RuntimeError(f"{path} is not built; run __graft_entry__.build() -- the MLP kernels have no CPU fallback")
+        import torch  # noqa: F401  (torch's HIP runtime first: it shares the soname libamdhip64.so.7)
+
         lib = ctypes.CDLL(str(path))
         lib.grg_mlp_linear_forward.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p]
         lib.grg_mlp_linear_backward.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

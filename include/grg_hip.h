@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GRG_HIP_ABI_VERSION 16
+#define GRG_HIP_ABI_VERSION 17
 
 /* uint32 words of the persistent job kernel's barrier buffer (160 KiB: its
  * counters sit on separate 256-byte lines). */
@@ -78,8 +78,9 @@ enum GrgHipDtype {
  * & 15) consecutive blocks, the eight XCDs' runs side by side, so every XCD's
  * L2 sees consecutive rows while all XCDs move through one window of rows
  * together (their shared sources stay in the memory-side cache). Blocks past
- * the last full window keep their order. Not combined with
- * GRG_HIP_FLAG_XCD_REMAP, PANEL or STREAM. No reference counterpart. */
+ * the last full window keep their order. Also orders the panels of a panel
+ * launch. Not combined with GRG_HIP_FLAG_XCD_REMAP or STREAM. No reference
+ * counterpart. */
 #define GRG_HIP_FLAG_XCD_RUNS 128
 #define GRG_HIP_XCD_RUN_SHIFT 8
 /* Per-row light path with one row per lane group (rows_per_group 1; merged
@@ -196,23 +197,25 @@ typedef struct GrgHipGatherArgs {
   /* Panel launch (GRG_HIP_FLAG_PANEL; no reference counterpart: an LDS-staged
    * form for rows that share many sources). Rows [row_begin, row_end) are cut
    * into num_panels panels of panel_rows consecutive rows (the last one cut at
-   * row_end); panel_rows * kl * es = 256, so each lane group owns one row of
-   * its panel and keeps that row's sums in registers. A panel's distinct
-   * sources are staged in passes of at most panel_staged rows: pass g (panel
+   * row_end), one workgroup of 1024 threads each: lane groups of kl lanes (es
+   * is 1) own rows_per_group consecutive rows each (1, 2 or 4, at most kl),
+   * so panel_rows = 1024 / kl * rows_per_group, and keep those rows' sums in
+   * registers. A panel's distinct sources are staged in passes: pass g (panel
    * p's passes are [pan_pass[p], pan_pass[p + 1])) copies the source rows
    * pan_src[pass_src[g] .. pass_src[g + 1]) (each (segment << 30) | source
-   * row; one column chunk of kl * cpl columns; every pass of a panel but its
-   * last stages exactly panel_staged rows) into LDS with its entry lists,
-   * and row i of the panel adds the staged rows pan_lidx[pass_ent[g] + j] for
-   * j in [pass_rowoff[g * (panel_rows + 1) + i], pass_rowoff[g * (panel_rows
-   * + 1) + i + 1]). The next pass's sources are loaded while the current one
-   * is summed. Heavy rows (as above) hold no panel entries and are produced by
-   * the heavy workgroups after the panels: grid = num_panels + num_heavy.
-   * kl * es is at least 2 (panel_rows at most 128).
-   * panel_staged is in [1, 256] with panel_staged * kl <= 1024;
-   * panel_max_entries bounds the entries of one pass (at most 16352) and
-   * panel_max_passes the passes of one panel; pan_lidx
-   * is 16-byte aligned and readable 16 bytes past its last entry. */
+   * row; at most panel_staged of them; one column chunk of kl * cpl columns)
+   * into LDS with its entry lists, and row i of the panel adds the staged rows
+   * pan_lidx[pass_ent[g] + j] for j in [pass_rowoff[g * (panel_rows + 1) + i],
+   * pass_rowoff[g * (panel_rows + 1) + i + 1]) (the first offset of a pass is
+   * 0; every offset and pass_ent entry is a multiple of 4; position 255 is
+   * padding and adds a zero row). unroll 4 keeps two passes in flight, unroll 8 three. Heavy rows (as
+   * above) hold no panel entries and are produced after the panels by the
+   * per-row kernel's heavy workgroups (a second launch on the stream).
+   * Combines with GRG_HIP_FLAG_XCD_REMAP or GRG_HIP_FLAG_XCD_RUNS (over the
+   * panels). panel_staged is in [1, 255] with panel_staged * kl <= 2048;
+   * panel_max_entries bounds the entries of one pass (at most 8160) and
+   * panel_max_passes the passes of one panel; pan_lidx is 16-byte aligned and
+   * readable 16 bytes past its last entry. */
   const int32_t* pan_pass;
   const int32_t* pass_src;
   const int32_t* pass_ent;

@@ -87,12 +87,14 @@ int grg_host_merged_keys(int64_t num_rows, int32_t nseg, const int32_t* const* s
  * at most once per row. Panel p is rows row_begin + p * panel_rows + [0,
  * panel_rows), cut at row_end. Rows of degree > heavy_threshold (when >= 0)
  * contribute no entries (their heavy workgroups produce them). The panel's
- * distinct keys, ascending, are staged max_staged at a time: pass q of a panel
- * stages keys [q * max_staged, (q + 1) * max_staged) of that list, and lists
- * for every row of the panel the entries whose key is in the pass, as 8-bit
- * positions in the pass, ascending. A row's entries are therefore summed in
- * ascending key order. panel_rows and max_staged are in [1, 256]; threads > 1
- * builds panels on that many host threads.
+ * distinct keys, ascending, are staged in passes of consecutive keys. Pass q
+ * lists for every row of the panel the entries whose key is in the pass, as
+ * 8-bit positions in the pass, ascending, padded to a multiple of four with
+ * position 255 (the kernel keeps staged row 255 at zero). A row's entries are
+ * therefore summed in ascending key order. A pass closes before its (max_staged
+ * + 1)-th key or before its padded entries would exceed max_pass_entries.
+ * panel_rows is in [1, 1024], max_staged in [1, 255], max_pass_entries in
+ * [4 * panel_rows, 65535]; threads > 1 builds panels on that many host threads.
  *
  * Returns an opaque handle (NULL on error, see grg_host_last_error), read with
  * grg_host_panels_sizes / grg_host_panels_copy and released with
@@ -107,7 +109,7 @@ int grg_host_merged_keys(int64_t num_rows, int32_t nseg, const int32_t* const* s
  *   src_list    [sources]: staged keys;
  *   lidx        [entries]: position of each entry's key in its pass. */
 void* grg_host_build_panels(int64_t row_begin, int64_t row_end, const int32_t* eptr, const uint32_t* keys, int32_t panel_rows,
-                            int32_t max_staged, int64_t heavy_threshold, int32_t threads);
+                            int32_t max_staged, int32_t max_pass_entries, int64_t heavy_threshold, int32_t threads);
 int grg_host_panels_sizes(const void* panels, int64_t* sizes);
 int grg_host_panels_copy(const void* panels, int32_t* pan_pass, int32_t* pass_src, int32_t* pass_ent, uint16_t* pass_rowoff,
                          uint32_t* src_list, uint8_t* lidx);

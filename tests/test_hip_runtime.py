@@ -227,28 +227,33 @@ def test_xcd_runs_flags_are_validated():
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("remap", [False, True])
-@pytest.mark.parametrize("kl", [None, 1])
-def test_panel_launch(art_a, oracle_a, dtype, remap, kl):
-    """The panel form (rows of a panel share passes of LDS-staged sources) on every
-    level of both sweeps, with heavy and split rows beside it, several passes per
-    panel (few staged sources per pass at small sizes) and one or several column
-    chunks: exact for +-1 inputs, FP64 within 1e-12, FP32c within 5e-7."""
+@pytest.mark.parametrize("order", ["plain", "remap", "runs"])
+@pytest.mark.parametrize("kl,panel_rows,unroll", [(None, 256, 8), (1, 1024, 4), (2, 512, 8), (None, 1024, 4)])
+def test_panel_launch(art_a, oracle_a, dtype, order, kl, panel_rows, unroll):
+    """The panel form (1024-thread workgroups; rows of a panel share passes of
+    LDS-staged sources) on every level of both sweeps, with heavy and split rows
+    beside it (their own launch), many passes per panel (at most 7 staged sources per
+    pass), 256-, 512- and 1024-row panels (1, 2 or 4 rows per lane group), two or
+    three passes in flight, every block order, and one or several column chunks:
+    exact for +-1 inputs, FP64 within 1e-12, FP32c within 5e-7."""
     from pygrgl_spmv.backends.hip import HipRuntime
     from pygrgl_spmv.backends.hip import ffi
 
+    from pygrgl_spmv.backends.hip.runtime import xcd_runs_flags
+
     layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
-    shape = {"unroll": 8, "rows_per_group": 1, "flags": ffi.FLAG_PANEL | (ffi.FLAG_XCD_REMAP if remap else 0)}
+    order_flags = {"plain": 0, "remap": ffi.FLAG_XCD_REMAP, "runs": xcd_runs_flags(1)}[order]
+    shape = {"unroll": unroll, "rows_per_group": 1, "flags": ffi.FLAG_PANEL | order_flags}
     if kl is not None:
         shape["kl"] = kl
     tol = FP64_TOL if dtype == np.float64 else FP32_TOL
     with HipRuntime(layout, fixed_shape=shape, split_min=40, split_chunk=16, panels=True, panel_min_mean=0.0,
-                    panel_min_rows=1) as rt:
+                    panel_min_rows=1, panel_rows=panel_rows, panel_staged=7) as rt:
         (grg,) = rt.grgs
         art = rt._artifacts[0]
         assert art.panels, "no level got panels"
-        assert max(m[0] for m in art.panels.values()) > 1
-        rng = np.random.default_rng(4410 + int(remap) + (kl or 0))
+        assert max(m[2] for m in art.panels.values()) > 3, "no panel with several passes"
+        rng = np.random.default_rng(4410 + len(order) + (kl or 0) + panel_rows)
         for k in (2, 16):
             for direction in ("up", "down"):
                 x = rng.choice(np.array([-1.0, 1.0]), size=(k, _cols(oracle_a, direction))).astype(dtype)

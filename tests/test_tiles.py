@@ -129,6 +129,9 @@ def test_merged_keys_follow_segment_order():
         merged_keys((eptr + 1).astype(np.int32), segs)
 
 
+PAD = 255  # entry position of the padding entries
+
+
 def _panel_sums(pn, values, row_begin, row_end):
     """Row sums through the panels, pass by pass, as the panel launch computes them."""
     R = pn.panel_rows
@@ -142,10 +145,13 @@ def _panel_sums(pn, values, row_begin, row_end):
             off = pn.pass_rowoff[g * (R + 1):(g + 1) * (R + 1)].astype(np.int64)
             ent = pn.lidx[pn.pass_ent[g]:pn.pass_ent[g + 1]].astype(np.int64)
             assert off[0] == 0 and off[-1] == ent.size and np.all(np.diff(off) >= 0)
+            assert np.all(off % 4 == 0) and int(pn.pass_ent[g]) % 4 == 0  # runs padded to multiples of 4
             for i in range(R):
                 r = p * R + i
-                pos = ent[off[i]:off[i + 1]]
-                assert np.all(np.diff(pos) > 0)  # ascending, each source once
+                run = ent[off[i]:off[i + 1]]
+                pos = run[run != PAD]
+                assert np.all(run[pos.size:] == PAD) and run.size - pos.size < 4  # padding at the end, < 4
+                assert np.all(np.diff(pos) > 0) and np.all(pos < keys.size)  # ascending, each source once
                 if r < row_end - row_begin:
                     out[r] += staged[pos].sum(axis=0)
                 else:
@@ -153,8 +159,10 @@ def _panel_sums(pn, values, row_begin, row_end):
     return out
 
 
-@pytest.mark.parametrize("panel_rows,max_staged,threads", [(64, 128, 1), (16, 7, 3), (1, 256, 2), (5, 1, 4)])
-def test_panels_reproduce_row_sums(panel_rows, max_staged, threads):
+@pytest.mark.parametrize("panel_rows,max_staged,max_entries,threads",
+                         [(64, 128, 8000, 1), (16, 7, 65535, 3), (1, 255, 4, 2), (5, 1, 20, 4), (1024, 255, 4096, 2),
+                          (256, 255, 1024, 3), (7, 255, 40, 1)])
+def test_panels_reproduce_row_sums(panel_rows, max_staged, max_entries, threads):
     from pygrgl_spmv.backends.hip.host import build_panels, merged_keys
 
     rng = np.random.default_rng(11)
@@ -171,20 +179,38 @@ def test_panels_reproduce_row_sums(panel_rows, max_staged, threads):
             plain[r] += v[i[p[r]:p[r + 1]]].sum(axis=0)
     heavy = 100
     for lo, hi in ((0, rows), (13, 250), (40, 40)):
-        pn = build_panels(eptr, keys, lo, hi, panel_rows=panel_rows, max_staged=max_staged, heavy_threshold=heavy,
-                          threads=threads)
+        pn = build_panels(eptr, keys, lo, hi, panel_rows=panel_rows, max_staged=max_staged,
+                          max_pass_entries=max_entries, heavy_threshold=heavy, threads=threads)
         assert pn.num_panels == -(-(hi - lo) // panel_rows)
         assert pn.max_passes == (int(np.diff(pn.pan_pass).max()) if pn.num_panels else 0)
-        # every pass of a panel but its last stages exactly max_staged sources
+        # passes are greedy runs of the panel's sources: each holds at most max_staged
+        # sources and max_entries entries, and the next source would break a limit
         for p in range(pn.num_panels):
-            n = np.diff(pn.pass_src[pn.pan_pass[p]:pn.pan_pass[p + 1] + 1])
-            assert np.all(n[:-1] == max_staged) and (n.size == 0 or 0 < n[-1] <= max_staged)
+            g0, g1 = int(pn.pan_pass[p]), int(pn.pan_pass[p + 1])
+            n_src = np.diff(pn.pass_src[g0:g1 + 1])
+            n_ent = np.diff(pn.pass_ent[g0:g1 + 1])
+            assert np.all((n_src > 0) & (n_src <= max_staged)) and np.all(n_ent <= max_entries)
+            keys_p = pn.src_list[pn.pass_src[g0]:pn.pass_src[g1]]
+            assert np.all(np.diff(keys_p.astype(np.int64)) > 0)  # distinct, ascending over the panel
+            # the next pass's first source would have broken a limit: its rows' padded runs grow
+            for j, g in enumerate(range(g0, g1 - 1)):
+                off = pn.pass_rowoff[g * (panel_rows + 1):(g + 1) * (panel_rows + 1)].astype(np.int64)
+                ent = pn.lidx[pn.pass_ent[g]:pn.pass_ent[g + 1]]
+                real = np.array([int(np.count_nonzero(ent[off[i]:off[i + 1]] != PAD)) for i in range(panel_rows)])
+                nxt = pn.src_list[pn.pass_src[g + 1]]
+                g2 = g + 1
+                off2 = pn.pass_rowoff[g2 * (panel_rows + 1):(g2 + 1) * (panel_rows + 1)].astype(np.int64)
+                ent2 = pn.lidx[pn.pass_ent[g2]:pn.pass_ent[g2 + 1]]
+                rows_nxt = [i for i in range(panel_rows) if 0 in ent2[off2[i]:off2[i + 1]]]
+                grow = sum(4 for i in rows_nxt if real[i] % 4 == 0)
+                assert n_src[j] == max_staged or n_ent[j] + grow > max_entries
+                del nxt
         got = _panel_sums(pn, values, lo, hi)
         want = plain[lo:hi].copy()
         want[deg[lo:hi] > heavy] = 0  # heavy rows are left to their heavy workgroups
         np.testing.assert_array_equal(got, want)
         # every non-heavy entry appears exactly once
-        assert int(pn.pass_ent[-1]) == int(deg[lo:hi][deg[lo:hi] <= heavy].sum())
+        assert int(np.count_nonzero(pn.lidx[:int(pn.pass_ent[-1])] != PAD)) == int(deg[lo:hi][deg[lo:hi] <= heavy].sum())
 
 
 def test_panels_reject_bad_arguments():
@@ -196,3 +222,9 @@ def test_panels_reject_bad_arguments():
         build_panels(eptr, keys, 0, 2, panel_rows=2, max_staged=4)
     with pytest.raises(RuntimeError, match="panel_rows"):
         build_panels(eptr, np.array([1, 3], dtype=np.uint32), 0, 2, panel_rows=0, max_staged=4)
+    with pytest.raises(RuntimeError, match="panel_rows"):
+        build_panels(eptr, np.array([1, 3], dtype=np.uint32), 0, 2, panel_rows=2048, max_staged=4)
+    with pytest.raises(RuntimeError, match="max_pass_entries"):
+        build_panels(eptr, np.array([1, 3], dtype=np.uint32), 0, 2, panel_rows=8, max_staged=4, max_pass_entries=31)
+    with pytest.raises(RuntimeError, match="max_staged"):
+        build_panels(eptr, np.array([1, 3], dtype=np.uint32), 0, 2, panel_rows=8, max_staged=256)
