@@ -394,10 +394,16 @@ class HipBatchedSweep:
     other. Every sweep writes private buffers, so ``entries[i]`` has its own
     ``inputs[i]`` / ``outputs[i]`` (and ``miss_inputs[i]``, ``init_vectors[i]``...)
     views, laid out as the prepared op's (k x cols, transposed views of row-major
-    arenas). Needs every block resident; tiled level launches run in their per-row
-    form inside job tables. Calling the object runs the batch on the runtime's
-    caller stream (capturable into a hipGraph); ``check()`` synchronizes and
-    raises if a persistent barrier wait ran past its bound."""
+    arenas). Tiled level launches run in their per-row form inside job tables.
+    Streamed blocks (layouts under a VRAM budget) are supported by the grouped form:
+    the phases run in segments, and between two segments the blocks streamed at that
+    point of every sweep are copied into their ring slots (pinned host -> device on
+    the slot's copy stream, after the last launch that read the slot) and
+    accumulated, as in a single op; such a batch runs eagerly only. The persistent
+    form needs every block resident. Calling the object runs the batch on the
+    runtime's caller stream (capturable into a hipGraph when every block is
+    resident); ``check()`` synchronizes and raises if a persistent barrier wait ran
+    past its bound."""
 
     def __init__(self, runtime: "HipRuntime", entries, *, persistent: bool, grid: int | None) -> None:
         torch = _torch()
@@ -408,6 +414,7 @@ class HipBatchedSweep:
         self.init_vectors, self.init_matrices = [], []
         self._buffers = []
         launches, phase_of = [], []
+        streamed = []  # (phase count before it, step, args): blocks streamed between phases
         dev = runtime.device
         tdt = runtime._torch_dtype()
         items_max = max([a.max_items for a in runtime._artifacts] + [1])
@@ -443,10 +450,15 @@ class HipBatchedSweep:
                                 partials=torch.zeros((items_max * k,), dtype=torch.float64, device=dev))
                 steps = runtime._build_steps(art, spec, bufs)
                 runtime._flush_tune_cache()
-                if any(it[0] != "batch" for it in steps):
-                    raise ValueError("a batched sweep needs every block of its GRG resident")
+                if self.persistent and any(it[0] != "batch" for it in steps):
+                    raise ValueError("a persistent batched sweep needs every block of its GRG resident; "
+                                     "run a layout with streamed blocks with persistent=False")
                 p = 0
-                for _, arr, n in steps:
+                for it in steps:
+                    if it[0] == "stream":
+                        streamed.append((p, it[1], it[2]))
+                        continue
+                    _, arr, n = it
                     for i in range(n):
                         a = ffi.GrgHipGatherArgs()
                         ctypes.pointer(a)[0] = arr[i]
@@ -478,6 +490,11 @@ class HipBatchedSweep:
             self._job_off_dev = torch.from_numpy(job_off).to(dev)
             self._blocks_dev = torch.from_numpy(blocks.copy() if blocks.size else np.zeros(1, np.int64)).to(dev)
             self._blocks_host = np.ascontiguousarray(blocks if blocks.size else np.zeros(1, np.int64))
+            # streamed blocks, in phase order (a stable sort keeps each sweep's own order),
+            # each with its own copy / compute events
+            streamed.sort(key=lambda t: t[0])
+            self._streamed = [(p, step, args, torch.cuda.Event(), torch.cuda.Event()) for p, step, args in streamed]
+            self._stream_launch = torch.cuda.Event() if streamed else None
             self._barrier = torch.zeros((ffi.BARRIER_WORDS,), dtype=torch.int32, device=dev)
             self._status = torch.zeros((1,), dtype=torch.int32, device=dev)
             max_grid = runtime._lib.jobs_max_grid(runtime._dtype_code, int(runtime._root_stream.cuda_stream))
@@ -487,13 +504,52 @@ class HipBatchedSweep:
                 raise ValueError(f"grid must be in [1, {max_grid}] (co-resident workgroups of the persistent kernel)")
         self.max_grid = max_grid
 
+    @property
+    def capturable(self) -> bool:
+        """Whether the batch may be captured into a hipGraph: False when it streams blocks."""
+        return not self._streamed
+
+    def _run_phases(self, p0: int, p1: int) -> None:
+        """Phases [p0, p1) of the grouped form (one launch per phase), or all of them
+        (persistent form), on the root stream."""
+        rt = self._runtime
+        if p1 <= p0:
+            return
+        job_off = self._job_off_dev.data_ptr() + 4 * p0
+        blocks = self._blocks_dev.data_ptr() + 8 * p0
+        rt._lib.jobs_run(self._jobs_dev.data_ptr(), job_off, blocks, self._blocks_host[p0:p1], rt._dtype_code, p1 - p0,
+                         self.persistent, self.grid, self._barrier.data_ptr(), self._status.data_ptr(),
+                         int(rt._root_stream.cuda_stream))
+
     def __call__(self) -> None:
+        torch = _torch()
         rt = self._runtime
         with rt._call_scope():
             with rt._caller_root_scope():
-                rt._lib.jobs_run(self._jobs_dev.data_ptr(), self._job_off_dev.data_ptr(), self._blocks_dev.data_ptr(),
-                                 self._blocks_host, rt._dtype_code, self.num_phases, self.persistent, self.grid,
-                                 self._barrier.data_ptr(), self._status.data_ptr(), int(rt._root_stream.cuda_stream))
+                if not self._streamed:
+                    self._run_phases(0, self.num_phases)
+                    return
+                if rt._lib.stream_capturing(int(rt._caller_stream.cuda_stream)):
+                    raise RuntimeError("HIP batched sweep: a batch that streams blocks through the pinned ring cannot be "
+                                       "captured into a hipGraph")
+                root = rt._root_stream
+                root_ptr = int(root.cuda_stream)
+                # every slot's first copy waits for the work enqueued before this batch
+                with torch.cuda.stream(root):
+                    self._stream_launch.record(root)
+                last_use = {}
+                done = 0
+                for p, step, args, copy_done, compute_done in self._streamed:
+                    self._run_phases(done, p)
+                    done = max(done, p)
+                    prev = last_use.get(step.slot, self._stream_launch)
+                    rt._copy_block_to_slot(step, prev, copy_done)
+                    root.wait_event(copy_done)
+                    rt._launch_streamed_block(args, root_ptr)
+                    with torch.cuda.stream(root):
+                        compute_done.record(root)
+                    last_use[step.slot] = compute_done
+                self._run_phases(done, self.num_phases)
 
     def tune_grid(self, candidates=None, *, reps: int = 3) -> dict:
         """Time the persistent launch at each candidate grid (default: a quarter, half,

@@ -13,7 +13,9 @@
   FP32c per-column error against the MKL sweep, reported and bounded.
 * C4, 22 per-chromosome GRGs over 200,000 samples with k from 1 to 64: under a
   budget that streams most blocks through a ring of 4 slots, every GRG's eager
-  up and down ops against the oracle on two of their columns; under the 32 GB
+  up and down ops against the oracle on two of their columns, and all 44 sweeps
+  as one grouped job-table batch (streamed blocks between its phases) against
+  the per-GRG ops; under the 32 GB
   budget (every block resident), the job-table forms (persistent and grouped,
   eager, captured and replayed) against the per-GRG ops.
 * C5, 200,000 x 2,000,000: the standardized kinship operator with FP32c sweeps,
@@ -212,6 +214,47 @@ def test_c4_streamed_ring_per_grg_ops_against_oracle():
             np.testing.assert_array_equal(grg.matmul(y, "down")[cols], O.matmul(g, y[cols], "down"))
         print(f"c4 streamed: 22 GRGs, up exact on +-1 inputs, down exact on +-1 and within {FP64_TOL:g} on N(0,1)",
               flush=True)
+
+
+def test_c4_streamed_batched_grouped_against_per_grg_ops():
+    """Under the same streaming budget and ring of 4 slots, the 22 GRGs' up and down sweeps
+    (mixed k) as ONE grouped job-table batch, its streamed blocks copied and accumulated
+    between phases: every output equal to the per-GRG op's on +-1 inputs, and within 1e-12
+    on N(0, 1) inputs."""
+    import torch
+
+    from pygrgl_spmv.chromosomes import mixed_k
+
+    paths = chromosome_artifacts()
+    budget = _c4_streamed_budget(paths)
+    with _runtime(paths, np.float64, k=64, budget=budget, ring=4) as rt:
+        assert any(not b.resident for a in rt.layout.artifacts for b in (*a.blocks_up, *a.blocks_down))
+        ks = [mixed_k(i) for i in range(len(paths))]
+        entries = [dict(grg=g, direction=d, k=k) for g, k in zip(rt.grgs, ks) for d in ("up", "down")]
+        batch = rt.prepare_batched_sweep(entries, persistent=False)
+        assert not batch.capturable
+        rng = np.random.default_rng(4410)
+        for kind in ("pm1", "normal"):
+            inputs = []
+            for e, view in zip(entries, batch.inputs):
+                cols = e["grg"].num_samples if e["direction"] == "up" else e["grg"].num_mutations
+                x = (rng.choice(np.array([-1.0, 1.0]), size=(e["k"], cols)) if kind == "pm1"
+                     else rng.standard_normal((e["k"], cols)))
+                view.copy_(torch.from_numpy(x))
+                inputs.append(x)
+            batch()
+            batch.check()
+            got = [out.cpu().numpy() for out in batch.outputs]
+            worst = 0.0
+            for e, x, out in zip(entries, inputs, got):
+                want = e["grg"].matmul(x, e["direction"])
+                if kind == "pm1":
+                    np.testing.assert_array_equal(out, want)
+                else:
+                    worst = max(worst, O.normwise_rel_err(out, want))
+            assert worst <= FP64_TOL, worst
+        print(f"c4 streamed batch: {len(entries)} sweeps in one grouped job table, {batch.num_phases} phases, "
+              f"{len(batch._streamed)} streamed block copies per run; exact on +-1, max rel err {worst:.2e}", flush=True)
 
 
 def test_c4_resident_batched_forms_against_per_grg_ops():

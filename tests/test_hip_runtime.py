@@ -780,6 +780,61 @@ def test_batched_sweep_of_several_grgs(artifact_dir, art_a, art_b, persistent):
                 _check(out.cpu().numpy(), w)
 
 
+@pytest.mark.parametrize("ring,residency", [(2, False), (3, True)])
+def test_batched_sweep_with_streamed_blocks(artifact_dir, art_a, art_b, ring, residency):
+    """The grouped batched form on layouts that stream blocks through the pinned ring
+    (every block streamed through 2 slots, or a budget that keeps part of them
+    resident with 3 slots): up and down sweeps of three GRGs with mixed k, the
+    streamed blocks copied and accumulated between phases, every output against the
+    oracle on two consecutive runs; the persistent form is refused and the batch is
+    not capturable."""
+    import torch
+
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    band = write_band_artifact(artifact_dir, "band-n64-batch-streamed", n=64, bandwidth=8)
+    arts = [art_a, art_b, band]
+    oracles = [O.load_artifact(p) for p in arts]
+    req = full_requirements(max_k_up=16, max_k_down=16)
+    full = build_hip_layout(arts, requirements=req, vram_budget_bytes=0)
+    budget = 0
+    if residency:
+        sparse = int(full.bytes_by_category.get("resident_sparse", 0))
+        budget = int(full.required_budget_for_full_residency) - sparse // 2
+    layout = build_hip_layout(arts, requirements=req, ring_buffer_size=ring, allow_residency=residency,
+                              vram_budget_bytes=budget)
+    n_streamed = sum(not b.resident for a in layout.artifacts for b in (*a.blocks_up, *a.blocks_down))
+    assert n_streamed > 0
+    rng = np.random.default_rng(3870 + ring)
+    with HipRuntime(layout) as rt:
+        ga, gb, gc = rt.grgs
+        oa, ob, oc = oracles
+        entries = [
+            dict(grg=ga, direction="up", k=5), dict(grg=ga, direction="down", k=16, use_miss=True),
+            dict(grg=gb, direction="up", k=1), dict(grg=gb, direction="down", k=7),
+            dict(grg=gc, direction="up", k=3), dict(grg=gc, direction="down", k=2),
+        ]
+        with pytest.raises(ValueError, match="persistent batched sweep needs every block"):
+            rt.prepare_batched_sweep(entries, persistent=True)
+        batch = rt.prepare_batched_sweep(entries, persistent=False)
+        assert not batch.capturable
+        for _ in range(2):
+            inputs = []
+            for e, view, g in zip(entries, batch.inputs, (oa, oa, ob, ob, oc, oc)):
+                x = rng.standard_normal((e["k"], _cols(g, e["direction"])))
+                view.copy_(torch.from_numpy(x))
+                inputs.append(x)
+            miss = rng.standard_normal((16, oa.num_mutations))
+            batch.miss_inputs[1].copy_(torch.from_numpy(miss))
+            batch()
+            batch.check()
+            want = [O.matmul(oa, inputs[0], "up"), O.matmul(oa, inputs[1], "down", miss=miss),
+                    O.matmul(ob, inputs[2], "up"), O.matmul(ob, inputs[3], "down"),
+                    O.matmul(oc, inputs[4], "up"), O.matmul(oc, inputs[5], "down")]
+            for out, w in zip(batch.outputs, want):
+                _check(out.cpu().numpy(), w)
+
+
 def test_persistent_full_grid_and_barrier_cost(artifact_dir, art_a, art_b):
     """The persistent launch at every co-resident workgroup (the grid where each XCD
     holds most members per sub-group): every output against the oracle on three
