@@ -259,8 +259,10 @@ GROUP_ROWS = 16       # rows of one group of the row-group launches (FP64 values
 
 
 def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequirements, dtype=np.float64) -> int:
-    """Upper bound of the tile tables of the tiled level launches, of the panels
-    of the panel launches and (FP64 values) of the row groups of the row-group launches.
+    """Upper bound of the tile tables of the tiled level launches and (FP64 values) of
+    the row groups of the row-group launches. (The panels of the panel launches, off by
+    default, are not reserved here: a runtime builds them only under a layout without
+    a VRAM budget.)
 
     Per sweep configuration (up; down; down with missing-data input): a uint16
     staged index per gather entry (plus 16 bytes of padding), at most one uint32
@@ -270,7 +272,6 @@ def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequiremen
     most one tile per row plus one per level).
     """
     N, L, E = scan.num_nodes, scan.num_levels, scan.num_edges
-    panels = N // PANEL_ROWS + L + 1
     entries = []  # (entries, merged keys kept: more than one segment)
     if pair.plan_up is not None:
         entries.append((E, False))
@@ -279,14 +280,6 @@ def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequiremen
         if req.need_down_miss_input:
             entries.append((E + scan.selector_mut_nnz + scan.selector_miss_nnz, True))
     tiles = sum(6 * e + (4 * e if keyed else 0) + 16 + 4 * (N + 1) + 8 * (N + L + 1) for e, keyed in entries)
-    # panels (no more than one per PANEL_ROWS rows plus one per level): a uint8 entry
-    # position and at most one uint32 staged source per entry, and per pass (at most
-    # one per PANEL_STAGED sources, one per PANEL_PASS_ENTRIES entries and two per
-    # panel) two int32 offsets and PANEL_ROWS + 1 uint16 row offsets
-    pan = 0
-    for e, _ in entries:
-        passes = e // PANEL_STAGED + e // PANEL_PASS_ENTRIES + 2 * panels
-        pan += 5 * e + 80 + 4 * (panels + 1) + (8 + 2 * (PANEL_ROWS + 1)) * (passes + 1)
     # row groups (no more than one per GROUP_ROWS rows plus one per level): at most one
     # uint32 key and uint16 mask per entry plus three padding entries per group, an
     # int64 offset and a uint16 store mask per group
@@ -295,7 +288,7 @@ def tile_bytes_for(scan: ArtifactScan, pair: HipPlanPair, req: RuntimeRequiremen
         groups = N // GROUP_ROWS + L + 1
         for e, _ in entries:
             grp += 6 * (e + 3 * groups) + 10 * (groups + 1) + 32
-    return int(tiles + pan + grp)
+    return int(tiles + grp)
 
 
 def plan_hip_layout(

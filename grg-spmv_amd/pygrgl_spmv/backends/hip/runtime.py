@@ -651,8 +651,9 @@ class HipRuntime:
         device node order sorts each run of ``degree_window`` consecutive nodes of a
         level (levels whose mean gather degree in the up sweep is at least
         ``degree_min_mean``) by degree, so the rows a wave sums in lockstep have
-        similar lengths (0 keeps the plain locality order). ``panels``: with the tile
-        tables, also build the panels of the panel launches (``panel_rows`` rows each: 256,
+        similar lengths (0 keeps the plain locality order). ``panels`` (off by default; only
+        under a layout without a VRAM budget): with the tile tables, also build the panels
+        of the panel launches (``panel_rows`` rows each: 256,
         512 or 1024; levels whose light rows average at least ``panel_min_mean`` entries,
         with at least ``panel_min_rows`` rows; passes of at most ``panel_staged`` sources)
         and let the tuner choose them. ``groups`` (FP64 values, with the tile tables):
@@ -698,6 +699,9 @@ class HipRuntime:
         self.panel_rows = int(panel_rows)
         if not 1 <= int(panel_staged) <= PANEL_STAGED:
             raise ValueError(f"panel_staged must be in [1, {PANEL_STAGED}]")
+        if panels and int(layout.vram_budget_bytes) > 0:
+            raise ValueError("panels=True builds panel tables beyond the layout's reservations: it needs a layout "
+                             "without a VRAM budget (vram_budget_bytes=0)")
         self.panel_staged = int(panel_staged)
         self.degree_window = int(degree_window)
         self.degree_min_mean = float(degree_min_mean)

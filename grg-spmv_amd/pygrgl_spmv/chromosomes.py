@@ -1,6 +1,7 @@
 """Several GRGs (one per chromosome) in one HIP runtime, captured into one hipGraph.
 
     PYTHONPATH=grg-spmv_amd python -m pygrgl_spmv.chromosomes [--chromosomes 22 --samples 200000 --budget-gb 32 --ring 4]
+    PYTHONPATH=grg-spmv_amd python -m pygrgl_spmv.chromosomes --streaming-budget
 
 The batched configuration of the benchmark suite: per-chromosome synthetic GRGs
 over the same samples with uneven mutation counts (U[20k, 400k]), one runtime
@@ -12,7 +13,9 @@ graph is replayed and each replay checked bit for bit against the eager run.
 The same sweeps then run from job tables (``HipRuntime.prepare_batched_sweep``):
 level p of every GRG in one phase, as ONE persistent launch per direction with
 grid-wide barriers between phases, and as one launch per phase; both are
-checked against the per-GRG ops, captured and replayed.
+checked against the per-GRG ops, captured and replayed. Under a budget that streams
+blocks (``--streaming-budget``) the ops run eagerly and the batch runs as grouped
+job tables with the streamed blocks copied and accumulated between phases.
 (Parity of multi-GRG runtimes with the CPU oracle is covered by the tests.)
 
 Prints progress on stderr and one JSON line on stdout.
@@ -36,6 +39,23 @@ def log(msg: str) -> None:
 def mixed_k(i: int) -> int:
     """Column counts cycling through 1..64 (powers of two and odd sizes)."""
     return (1, 3, 8, 16, 5, 32, 64, 2, 12, 48, 7)[i % 11]
+
+
+def streaming_budget(artifacts, *, ring: int = 4, fraction: float = 0.125) -> int:
+    """A VRAM budget ``fraction`` of the way from the all-streamed layout's bytes to full
+    residency: the planner keeps the largest blocks resident and streams the rest through
+    the ring (the 32 GB of the configuration hold every block of these GRGs)."""
+    from pygrgl_spmv.backends.hip import HipPlanPair, plan_hip_layout
+    from pygrgl_spmv.grg import RuntimeRequirements
+
+    req = RuntimeRequirements(max_k_up=64, max_k_down=64, need_down_miss_input=False, need_up_miss_output=False,
+                              need_init_vector=False, need_init_matrix=False, need_init_xtx=False)
+    kw = dict(artifacts=[Path(a) for a in artifacts], pair=HipPlanPair.default(), dtype=np.float64, requirements=req,
+              device=0, stream=0)
+    full = plan_hip_layout(vram_budget_bytes=0, ring_buffer_size=0, **kw)
+    streamed = plan_hip_layout(vram_budget_bytes=0, ring_buffer_size=int(ring), allow_residency=False, **kw)
+    lo, hi = int(streamed.bytes_total), int(full.required_budget_for_full_residency)
+    return lo + int((hi - lo) * float(fraction))
 
 
 def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: int = 3, seed: int = 4,
@@ -127,6 +147,45 @@ def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: i
             out["graph_edge_updates_per_s"] = edges / (min(times) / 1e3)
         log(f"eager {out['eager_ms']:.2f} ms" + (f", graph replay {out['graph_replay_ms']:.2f} ms" if capturable else
                                                   ", not captured (streamed blocks)"))
+        if not capturable:
+            # the same 44 sweeps as grouped job tables, the streamed blocks copied and
+            # accumulated between phases (eager only)
+            tol = 1e-12 if np.dtype(dtype) == np.float64 else 5e-7
+            ups = rt.prepare_batched_sweep([dict(grg=g, direction="up", k=mixed_k(i)) for i, g in enumerate(grgs)],
+                                           persistent=False)
+            dns = rt.prepare_batched_sweep([dict(grg=g, direction="down", k=mixed_k(i)) for i, g in enumerate(grgs)],
+                                           persistent=False)
+
+            def batched_streamed():
+                for v, x in zip(ups.inputs, xs):
+                    v.copy_(x)
+                for v, y in zip(dns.inputs, ys):
+                    v.copy_(y)
+                ups()
+                dns()
+
+            with torch.cuda.stream(cap):
+                batched_streamed()
+            torch.cuda.synchronize()
+            err = 0.0
+            for a, b in zip((*ups.outputs, *dns.outputs), (*eager_up, *eager_dn)):
+                a64, b64 = a.cpu().double(), b.double()
+                err = max(err, float((a64 - b64).norm() / max(float(b64.norm()), 1e-300)))
+            if err > tol:
+                raise AssertionError(f"grouped streamed batched sweep differs from the per-GRG ops: {err:.2e}")
+            with torch.cuda.stream(cap):
+                ev[0].record(cap)
+                batched_streamed()
+                ev[1].record(cap)
+            torch.cuda.synchronize()
+            res = {"phases_up": ups.num_phases, "phases_down": dns.num_phases, "jobs": ups.num_jobs + dns.num_jobs,
+                   "streamed_block_copies": len(ups._streamed) + len(dns._streamed),
+                   "eager_ms": ev[0].elapsed_time(ev[1]), "max_rel_err_vs_per_grg_ops": err}
+            res["eager_edge_updates_per_s"] = edges / (res["eager_ms"] / 1e3)
+            out["batched_grouped_streamed"] = res
+            log(f"batched (grouped, streamed): {res['jobs']} jobs, {res['streamed_block_copies']} block copies, eager "
+                f"{res['eager_ms']:.2f} ms, rel err vs per-GRG ops {err:.1e}")
+            del ups, dns
         if capturable:
             # the same 44 sweeps from job tables: all GRGs' level p in one phase
             tol = 1e-12 if np.dtype(dtype) == np.float64 else 5e-7
@@ -206,6 +265,8 @@ def main() -> int:
     ap.add_argument("--chromosomes", type=int, default=22)
     ap.add_argument("--samples", type=int, default=200_000)
     ap.add_argument("--budget-gb", type=float, default=32.0)
+    ap.add_argument("--streaming-budget", action="store_true",
+                    help="instead of --budget-gb: a budget an eighth of the way from all-streamed to full residency")
     ap.add_argument("--ring", type=int, default=4)
     ap.add_argument("--artifact-dir", default="/tmp/grg_chrom")
     ap.add_argument("--fp32c", action="store_true")
@@ -223,8 +284,8 @@ def main() -> int:
             write_artifact(cfg, p)
         arts.append(p)
     log(f"{len(arts)} artifacts ready in {time.perf_counter() - t:.1f}s")
-    res = run(arts, budget_bytes=int(args.budget_gb * 1e9), ring=args.ring,
-              dtype=np.float32 if args.fp32c else np.float64)
+    budget = streaming_budget(arts, ring=args.ring) if args.streaming_budget else int(args.budget_gb * 1e9)
+    res = run(arts, budget_bytes=budget, ring=args.ring, dtype=np.float32 if args.fp32c else np.float64)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -173,18 +173,11 @@ def test_c3_wide_k_fp32c_error_against_fp64_oracle(c3_reference):
 
 
 def _c4_streamed_budget(paths) -> int:
-    """A budget between the all-streamed layout's and full residency's (an eighth of the way):
+    """A budget an eighth of the way from the all-streamed layout's bytes to full residency:
     the planner keeps the largest blocks resident and streams the rest through the ring."""
-    from pygrgl_spmv.backends.hip import HipPlanPair, plan_hip_layout
-    from pygrgl_spmv.grg import RuntimeRequirements
+    from pygrgl_spmv.chromosomes import streaming_budget
 
-    req = RuntimeRequirements(max_k_up=64, max_k_down=64, need_down_miss_input=False, need_up_miss_output=False,
-                              need_init_vector=False, need_init_matrix=False, need_init_xtx=False)
-    kw = dict(artifacts=list(paths), pair=HipPlanPair.default(), dtype=np.float64, requirements=req, device=0, stream=0)
-    full = plan_hip_layout(vram_budget_bytes=0, ring_buffer_size=0, **kw)
-    streamed = plan_hip_layout(vram_budget_bytes=0, ring_buffer_size=4, allow_residency=False, **kw)
-    lo, hi = int(streamed.bytes_total), int(full.required_budget_for_full_residency)
-    return lo + (hi - lo) // 8
+    return streaming_budget(paths, ring=4)
 
 
 def test_c4_streamed_ring_per_grg_ops_against_oracle():

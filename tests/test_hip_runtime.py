@@ -241,7 +241,8 @@ def test_panel_launch(art_a, oracle_a, dtype, order, kl, panel_rows, unroll):
 
     from pygrgl_spmv.backends.hip.runtime import xcd_runs_flags
 
-    layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16))
+    layout = build_hip_layout([art_a], dtype=dtype, requirements=full_requirements(max_k_up=16, max_k_down=16),
+                              vram_budget_bytes=0)
     order_flags = {"plain": 0, "remap": ffi.FLAG_XCD_REMAP, "runs": xcd_runs_flags(1)}[order]
     shape = {"unroll": unroll, "rows_per_group": 1, "flags": ffi.FLAG_PANEL | order_flags}
     if kl is not None:
@@ -269,10 +270,14 @@ def test_panel_launch(art_a, oracle_a, dtype, order, kl, panel_rows, unroll):
 
 def test_panel_launch_tuned(art_a, oracle_a):
     """The tuner with panels on every level next to the other forms: whatever it picks
-    matches the oracle."""
+    matches the oracle. Panels need a layout without a VRAM budget."""
     from pygrgl_spmv.backends.hip import HipRuntime
 
-    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=16, max_k_down=16))
+    budgeted = build_hip_layout([art_a], requirements=full_requirements(max_k_up=16, max_k_down=16))
+    with pytest.raises(ValueError, match="without a VRAM budget"):
+        HipRuntime(budgeted, panels=True)
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=16, max_k_down=16), vram_budget_bytes=0)
     with HipRuntime(layout, panels=True, panel_min_mean=0.0, panel_min_rows=1) as rt:
         (grg,) = rt.grgs
         rng = np.random.default_rng(4420)
