@@ -187,33 +187,41 @@ def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: i
                 f"{res['eager_ms']:.2f} ms, rel err vs per-GRG ops {err:.1e}")
             del ups, dns
         if capturable:
-            # the same 44 sweeps from job tables: all GRGs' level p in one phase
+            # the same 44 sweeps from job tables: all GRGs' level p in one phase, the up and
+            # down sweeps as two batches, or as ONE batch (every sweep writes private
+            # buffers, so an up and a down sweep share phases and barriers)
             tol = 1e-12 if np.dtype(dtype) == np.float64 else 5e-7
-            for mode, persistent in (("persistent", True), ("grouped", False)):
-                ups = rt.prepare_batched_sweep([dict(grg=g, direction="up", k=mixed_k(i)) for i, g in enumerate(grgs)],
-                                               persistent=persistent)
-                dns = rt.prepare_batched_sweep([dict(grg=g, direction="down", k=mixed_k(i)) for i, g in enumerate(grgs)],
-                                               persistent=persistent)
+            ins = [*xs, *ys]
+            want = (*eager_up, *eager_dn)
+            up_entries = [dict(grg=g, direction="up", k=mixed_k(i)) for i, g in enumerate(grgs)]
+            dn_entries = [dict(grg=g, direction="down", k=mixed_k(i)) for i, g in enumerate(grgs)]
+            for mode, persistent, combined in (("persistent", True, False), ("grouped", False, False),
+                                               ("persistent_one_batch", True, True), ("grouped_one_batch", False, True)):
+                if combined:
+                    batches = [rt.prepare_batched_sweep(up_entries + dn_entries, persistent=persistent)]
+                else:
+                    batches = [rt.prepare_batched_sweep(up_entries, persistent=persistent),
+                               rt.prepare_batched_sweep(dn_entries, persistent=persistent)]
+                views = [v for b in batches for v in b.inputs]
+                outputs = [o for b in batches for o in b.outputs]
                 grid_ms = {}
                 if persistent and tune_grid:
-                    grid_ms = {"up": ups.tune_grid(), "down": dns.tune_grid()}
-                    log(f"persistent grids (ms per sweep): up {grid_ms['up']} -> {ups.grid}, down {grid_ms['down']} -> {dns.grid}")
+                    grid_ms = [b.tune_grid() for b in batches]
+                    log(f"{mode}: persistent grids (ms per batch) {grid_ms} -> {[b.grid for b in batches]}")
 
                 def batched():
-                    for v, x in zip(ups.inputs, xs):
+                    for v, x in zip(views, ins):
                         v.copy_(x)
-                    for v, y in zip(dns.inputs, ys):
-                        v.copy_(y)
-                    ups()
-                    dns()
+                    for b in batches:
+                        b()
 
                 with torch.cuda.stream(cap):
                     batched()
                 torch.cuda.synchronize()
-                ups.check()
-                dns.check()
+                for b in batches:
+                    b.check()
                 err = 0.0
-                for a, b in zip((*ups.outputs, *dns.outputs), (*eager_up, *eager_dn)):
+                for a, b in zip(outputs, want):
                     a64 = a.cpu().double()
                     b64 = b.double()
                     err = max(err, float((a64 - b64).norm() / max(float(b64.norm()), 1e-300)))
@@ -224,17 +232,17 @@ def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: i
                     batched()
                     ev[1].record(cap)
                 torch.cuda.synchronize()
-                res = {"phases_up": ups.num_phases, "phases_down": dns.num_phases, "jobs": ups.num_jobs + dns.num_jobs,
-                       "grid": [ups.grid, dns.grid] if persistent else None, "grid_ms": grid_ms,
-                       "eager_ms": ev[0].elapsed_time(ev[1]),
-                       "max_rel_err_vs_per_grg_ops": err}
-                eager_b = [o.cpu().clone() for o in (*ups.outputs, *dns.outputs)]
+                res = {"batches": len(batches), "phases": [b.num_phases for b in batches],
+                       "jobs": sum(b.num_jobs for b in batches),
+                       "grid": [b.grid for b in batches] if persistent else None, "grid_ms": grid_ms,
+                       "eager_ms": ev[0].elapsed_time(ev[1]), "max_rel_err_vs_per_grg_ops": err}
+                eager_b = [o.cpu().clone() for o in outputs]
                 graph = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(graph, stream=cap):
                     batched()
                 times = []
                 for rep in range(int(replays)):
-                    for o in (*ups.outputs, *dns.outputs):
+                    for o in outputs:
                         o.zero_()
                     torch.cuda.synchronize()
                     with torch.cuda.stream(cap):
@@ -243,19 +251,20 @@ def run(artifacts, *, budget_bytes: int, ring: int, dtype=np.float64, replays: i
                         ev[1].record(cap)
                     torch.cuda.synchronize()
                     times.append(ev[0].elapsed_time(ev[1]))
-                    ups.check()
-                    if not all(torch.equal(o.cpu(), b) for o, b in zip((*ups.outputs, *dns.outputs), eager_b)):
+                    for b in batches:
+                        b.check()
+                    if not all(torch.equal(o.cpu(), e) for o, e in zip(outputs, eager_b)):
                         raise AssertionError(f"{mode} batched graph replay {rep} differs from its eager run")
                 res["graph_replay_ms"] = min(times)
                 res["graph_edge_updates_per_s"] = edges / (min(times) / 1e3)
                 if persistent:
-                    res.update(ups.barrier_costs())
-                    log(f"persistent kernel at {ups.grid} workgroups: grid barrier {res['barrier_us']:.2f} us, "
+                    res.update(batches[0].barrier_costs())
+                    log(f"{mode}: persistent kernel at {batches[0].grid} workgroups: grid barrier {res['barrier_us']:.2f} us, "
                         f"one empty phase (reset, launch, registration) {res['one_phase_us']:.2f} us")
                 out[f"batched_{mode}"] = res
-                log(f"batched ({mode}): {res['jobs']} jobs in {res['phases_up']} + {res['phases_down']} phases, eager "
-                    f"{res['eager_ms']:.2f} ms, graph replay {res['graph_replay_ms']:.2f} ms, rel err vs per-GRG ops {err:.1e}")
-                del graph, ups, dns
+                log(f"batched ({mode}): {res['jobs']} jobs in {res['phases']} phases, eager {res['eager_ms']:.2f} ms, "
+                    f"graph replay {res['graph_replay_ms']:.2f} ms, rel err vs per-GRG ops {err:.1e}")
+                del graph, batches
     out["seconds_total"] = time.perf_counter() - t0
     return out
 

@@ -252,14 +252,15 @@ def test_c4_streamed_batched_grouped_against_per_grg_ops():
 
 def test_c4_resident_batched_forms_against_per_grg_ops():
     """The configuration's 32 GB budget keeps every block resident: every op captured in one
-    graph and replayed bit for bit; the persistent and grouped job-table forms within 1e-12 of
-    the per-GRG ops, captured and replayed bit for bit (``chromosomes.run`` raises otherwise)."""
+    graph and replayed bit for bit; the persistent and grouped job-table forms (up and down
+    sweeps as two batches, or all 44 sweeps as one) within 1e-12 of the per-GRG ops, captured
+    and replayed bit for bit (``chromosomes.run`` raises otherwise)."""
     from pygrgl_spmv.chromosomes import run
 
     res = run(chromosome_artifacts(), budget_bytes=32_000_000_000, ring=4, replays=2, autotune=False)
     print({k: v for k, v in res.items() if k not in ("edges", "ks")}, flush=True)
     assert res["streamed_blocks"] == 0 and res["captured"] and res["graph_replays_checked"] == 2
-    for mode in ("persistent", "grouped"):
+    for mode in ("persistent", "grouped", "persistent_one_batch", "grouped_one_batch"):
         assert res[f"batched_{mode}"]["max_rel_err_vs_per_grg_ops"] <= FP64_TOL
 
 
