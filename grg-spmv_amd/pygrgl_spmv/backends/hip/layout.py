@@ -118,6 +118,8 @@ class HipLayout:
     heavy_threshold: int = HEAVY_THRESHOLD
     schedule_capacity: tuple[int, ...] = ()
     tiles: bool = False
+    # True when the caller capped device memory; an uncapped layout stores its full-residency size above.
+    budgeted: bool = False
 
 
 def _resolve_artifacts(artifacts) -> tuple[Path, ...]:
@@ -449,6 +451,7 @@ def plan_hip_layout(
         budget_items=tuple(i for i in items if i.nbytes > 0), required_budget_for_full_residency=int(required_full),
         bytes_by_category=by_cat, bytes_total=int(sum(i.nbytes for i in items)),
         heavy_threshold=HEAVY_THRESHOLD, schedule_capacity=tuple(heavy_caps), tiles=bool(tiles),
+        budgeted=budget > 0,
     )
 
 

@@ -699,7 +699,7 @@ class HipRuntime:
         self.panel_rows = int(panel_rows)
         if not 1 <= int(panel_staged) <= PANEL_STAGED:
             raise ValueError(f"panel_staged must be in [1, {PANEL_STAGED}]")
-        if panels and int(layout.vram_budget_bytes) > 0:
+        if panels and layout.budgeted:
             raise ValueError("panels=True builds panel tables beyond the layout's reservations: it needs a layout "
                              "without a VRAM budget (vram_budget_bytes=0)")
         self.panel_staged = int(panel_staged)

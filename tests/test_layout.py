@@ -149,7 +149,9 @@ def test_zero_budget_sentinel_and_tight_ring(band, budget, ring, case):
         assert _keys(lay, False) == ()
         assert lay.allocated_ring_buffer_size == 0
         assert lay.vram_budget_bytes == lay.required_budget_for_full_residency == lay.bytes_total
+        assert not lay.budgeted
     else:
+        assert lay.budgeted
         assert lay.allocated_ring_buffer_size == 1
         assert lay.bytes_total <= lay.vram_budget_bytes
         assert lay.bytes_total < lay.required_budget_for_full_residency
