@@ -113,6 +113,59 @@ void locality_order(int64_t N, int64_t S, int32_t L, const int64_t* off, const G
   }
 }
 
+// Min-child order inside windows of a level (see grg_host_cluster_order).
+void cluster_order(int64_t N, int32_t L, const int64_t* off, const GrgHostBlock* blocks, int32_t nb, int64_t window,
+                   int64_t degree_window, double min_mean, int64_t min_rows, int64_t* new_pos) {
+  if (N < 0 || L < 0 || off[0] != 0 || off[L] != N) throw std::invalid_argument("level offsets do not cover the nodes");
+  if (window < 2) throw std::invalid_argument("window must be >= 2");
+  if (degree_window < 0) throw std::invalid_argument("degree_window < 0");
+  for (int64_t v = 0; v < N; ++v)
+    if (new_pos[v] < 0 || new_pos[v] >= N) throw std::invalid_argument("new_pos out of range");
+  for (int32_t b = 0; b < nb; ++b)
+    if (blocks[b].dst_level <= blocks[b].src_level || blocks[b].dst_level >= L) throw std::invalid_argument("block levels out of order");
+  std::vector<int32_t> order(nb);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return blocks[a].dst_level < blocks[b].dst_level; });
+  std::vector<int64_t> mn, dg, ids;
+  size_t bi = 0;
+  for (int32_t l = 1; l < L; ++l) {
+    const int64_t lo = off[l], hi = off[l + 1], n = hi - lo;
+    // the level's blocks (children at lower levels, whose positions are final by now)
+    mn.assign(n, INT64_MAX);
+    dg.assign(n, 0);
+    for (; bi < order.size() && blocks[order[bi]].dst_level == l; ++bi) {
+      const GrgHostBlock& B = blocks[order[bi]];
+      const int64_t src0 = off[B.src_level], src1 = off[B.src_level + 1];
+      for (int64_t r = 0; r < n; ++r) {
+        for (int64_t j = B.indptr[r]; j < B.indptr[r + 1]; ++j) {
+          const int64_t c = src0 + B.indices[j];
+          if (c < src0 || c >= src1) throw std::invalid_argument("block column out of range");
+          mn[r] = std::min(mn[r], new_pos[c]);
+        }
+        dg[r] += B.indptr[r + 1] - B.indptr[r];
+      }
+    }
+    int64_t e = 0;
+    for (int64_t r = 0; r < n; ++r) e += dg[r];
+    if (n < std::max<int64_t>(2, min_rows) || (double)e < min_mean * (double)n) continue;
+    // rows (level-local) in their current order
+    ids.resize(n);
+    for (int64_t r = 0; r < n; ++r) ids[new_pos[lo + r] - lo] = r;
+    for (int64_t w = 0; w < n; w += window) {
+      const int64_t we = std::min(n, w + window);
+      // rows whose smallest child comes first lead; rows sharing their smallest child end up side by side
+      std::stable_sort(ids.begin() + w, ids.begin() + we, [&](int64_t a, int64_t b) { return mn[a] < mn[b]; });
+      if (degree_window > 1) {
+        for (int64_t u = w; u < we; u += degree_window) {
+          const int64_t ue = std::min(we, u + degree_window);
+          std::stable_sort(ids.begin() + u, ids.begin() + ue, [&](int64_t a, int64_t b) { return dg[a] < dg[b]; });
+        }
+      }
+    }
+    for (int64_t i = 0; i < n; ++i) new_pos[lo + ids[i]] = lo + i;
+  }
+}
+
 void build_tiles(int64_t R, int32_t L, const int64_t* off, int32_t nseg, const int32_t* const* sp,
                  const int32_t* const* si, const int64_t* src_rows, const int32_t* eptr, const int64_t* heavy,
                  int32_t max_staged, int32_t max_rows, int32_t max_entries, int32_t* tile_row, int32_t* tile_src,
@@ -399,6 +452,22 @@ int grg_host_locality_order(int64_t num_nodes, int64_t num_samples, int32_t num_
   try {
     locality_order(num_nodes, num_samples, num_levels, level_offsets, blocks, num_blocks, degree_window, degree_min_mean,
                    new_pos);
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 2;
+  }
+}
+
+int grg_host_cluster_order(int64_t num_nodes, int32_t num_levels, const int64_t* level_offsets, const GrgHostBlock* blocks,
+                           int32_t num_blocks, int64_t window, int64_t degree_window, double min_mean, int64_t min_rows,
+                           int64_t* new_pos) {
+  if (!level_offsets || !new_pos || (num_blocks > 0 && !blocks)) {
+    g_last_error = "null argument";
+    return 1;
+  }
+  try {
+    cluster_order(num_nodes, num_levels, level_offsets, blocks, num_blocks, window, degree_window, min_mean, min_rows, new_pos);
     return 0;
   } catch (const std::exception& e) {
     g_last_error = e.what();

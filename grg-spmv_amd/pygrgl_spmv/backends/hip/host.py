@@ -28,6 +28,9 @@ def _lib():
         lib.grg_host_locality_order.argtypes = [c_int64, c_int64, c_int32, c_void_p, ctypes.POINTER(GrgHostBlock), c_int32,
                                                 c_int64, ctypes.c_double, c_void_p]
         lib.grg_host_locality_order.restype = ctypes.c_int
+        lib.grg_host_cluster_order.argtypes = [c_int64, c_int32, c_void_p, ctypes.POINTER(GrgHostBlock), c_int32, c_int64,
+                                               c_int64, ctypes.c_double, c_int64, c_void_p]
+        lib.grg_host_cluster_order.restype = ctypes.c_int
         lib.grg_host_build_tiles.argtypes = [c_int64, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                              c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                              c_void_p, c_void_p]
@@ -56,6 +59,17 @@ def _lib():
     return _LIB
 
 
+def _host_blocks(blocks):
+    keep = []
+    arr = (GrgHostBlock * max(1, len(blocks)))()
+    for i, (d, s, ptr, idx) in enumerate(blocks):
+        ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        keep.append((ptr, idx))
+        arr[i] = GrgHostBlock(int(d), int(s), int(idx.size), ptr.ctypes.data, idx.ctypes.data)
+    return arr, keep
+
+
 def locality_order(num_nodes: int, num_samples: int, level_offsets: np.ndarray, blocks, *, degree_window: int = 0,
                    degree_min_mean: float = 0.0) -> np.ndarray:
     """``new_pos[old]`` for blocks given as (dst, src, indptr int64, indices int32) tuples.
@@ -64,18 +78,30 @@ def locality_order(num_nodes: int, num_samples: int, level_offsets: np.ndarray, 
     (levels above 0 with mean child count >= ``degree_min_mean``) by child count."""
     lib = _lib()
     off = np.ascontiguousarray(level_offsets, dtype=np.int64)
-    keep = []
-    arr = (GrgHostBlock * max(1, len(blocks)))()
-    for i, (d, s, ptr, idx) in enumerate(blocks):
-        ptr = np.ascontiguousarray(ptr, dtype=np.int64)
-        idx = np.ascontiguousarray(idx, dtype=np.int32)
-        keep.append((ptr, idx))
-        arr[i] = GrgHostBlock(int(d), int(s), int(idx.size), ptr.ctypes.data, idx.ctypes.data)
+    arr, keep = _host_blocks(blocks)
     out = np.empty(int(num_nodes), dtype=np.int64)
     rc = lib.grg_host_locality_order(int(num_nodes), int(num_samples), int(off.size - 1), off.ctypes.data, arr, len(blocks),
                                      int(degree_window), float(degree_min_mean), out.ctypes.data)
     if rc != 0:
         raise RuntimeError(f"grg_host_locality_order failed: status {rc}: {lib.grg_host_last_error().decode()}")
+    return out
+
+
+def cluster_order(new_pos: np.ndarray, level_offsets: np.ndarray, blocks, *, window: int, degree_window: int = 0,
+                  min_mean: float = 0.0, min_rows: int = 0) -> np.ndarray:
+    """``new_pos`` refined by the min-child order (``grg_host_cluster_order``): inside
+    windows of ``window`` consecutive positions of every level above 0 with at least
+    ``min_rows`` nodes and a mean child count of at least ``min_mean``, nodes ordered by
+    the position of their first child; then, with ``degree_window > 1``, aligned runs of
+    that many positions by child count. Returns a new array; the input is not modified."""
+    lib = _lib()
+    off = np.ascontiguousarray(level_offsets, dtype=np.int64)
+    arr, keep = _host_blocks(blocks)
+    out = np.array(new_pos, dtype=np.int64, copy=True)
+    rc = lib.grg_host_cluster_order(int(out.size), int(off.size - 1), off.ctypes.data, arr, len(blocks), int(window),
+                                    int(degree_window), float(min_mean), int(min_rows), out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"grg_host_cluster_order failed: status {rc}: {lib.grg_host_last_error().decode()}")
     return out
 
 
@@ -174,7 +200,7 @@ def merged_keys(eptr: np.ndarray, segments) -> np.ndarray:
     return ekey
 
 
-__all__ = ["LIB_PATH", "LIDX_PAD", "Tiles", "build_tiles", "locality_order", "merged_keys"]
+__all__ = ["LIB_PATH", "LIDX_PAD", "Tiles", "build_tiles", "cluster_order", "locality_order", "merged_keys"]
 
 
 @dataclass

@@ -83,6 +83,9 @@ GROUP_MIN_ROWS = 256    # ... nor levels with fewer rows
 GROUP_MAX_FILL = 0.9    # ... nor levels whose groups keep more than this share of the entries
 DEGREE_WINDOW = 4096    # nodes per degree-sorted run of the device node order
 DEGREE_MIN_MEAN = 6.0
+CLUSTER_WINDOW = 0          # nodes per min-child-ordered window of the device node order (0: off)
+CLUSTER_DEGREE_WINDOW = 0   # aligned runs inside those windows ordered by degree (0: none)
+CLUSTER_MIN_ROWS = 65536    # levels with fewer nodes keep the locality order
 # Run lengths (log2 of light blocks) the tuner tries for GRG_HIP_FLAG_XCD_RUNS.
 XCD_RUN_LOGS = (4, 5, 6, 8, 9, 10, 12)
 # The tuner's last stage: the fastest few distinct launches re-timed, interleaved, in
@@ -630,7 +633,9 @@ class HipRuntime:
                  split_chunk: int = SPLIT_CHUNK, fixed_shape: dict | None = None, tiles: bool = True,
                  tile_max_staged: int = TILE_MAX_STAGED, tile_max_rows: int = TILE_MAX_ROWS,
                  tile_max_entries: int = TILE_MAX_ENTRIES, degree_window: int = DEGREE_WINDOW,
-                 degree_min_mean: float = DEGREE_MIN_MEAN, panels: bool = False, panel_min_mean: float = PANEL_MIN_MEAN,
+                 degree_min_mean: float = DEGREE_MIN_MEAN, cluster_window: int = CLUSTER_WINDOW,
+                 cluster_degree_window: int = CLUSTER_DEGREE_WINDOW, cluster_min_rows: int = CLUSTER_MIN_ROWS,
+                 panels: bool = False, panel_min_mean: float = PANEL_MIN_MEAN,
                  panel_min_rows: int = PANEL_MIN_ROWS, panel_rows: int = PANEL_ROWS, panel_staged: int = PANEL_STAGED,
                  groups: bool = True,
                  group_min_mean: float = GROUP_MIN_MEAN,
@@ -651,7 +656,12 @@ class HipRuntime:
         device node order sorts each run of ``degree_window`` consecutive nodes of a
         level (levels whose mean gather degree in the up sweep is at least
         ``degree_min_mean``) by degree, so the rows a wave sums in lockstep have
-        similar lengths (0 keeps the plain locality order). ``panels`` (off by default; only
+        similar lengths (0 keeps the plain locality order). ``cluster_window`` (> 1 turns it
+        on): then, on levels with at least ``cluster_min_rows`` nodes and that mean degree,
+        windows of ``cluster_window`` consecutive nodes are ordered by the position of each
+        node's first child, so rows that share it sit side by side (a workgroup's rows gather
+        more of the same sources), and aligned runs of ``cluster_degree_window`` nodes inside
+        them by degree (``grg_host_cluster_order``). ``panels`` (off by default; only
         under a layout without a VRAM budget): with the tile tables, also build the panels
         of the panel launches (``panel_rows`` rows each: 256,
         512 or 1024; levels whose light rows average at least ``panel_min_mean`` entries,
@@ -705,6 +715,11 @@ class HipRuntime:
         self.panel_staged = int(panel_staged)
         self.degree_window = int(degree_window)
         self.degree_min_mean = float(degree_min_mean)
+        if int(cluster_window) < 0 or int(cluster_degree_window) < 0 or int(cluster_min_rows) < 0:
+            raise ValueError("cluster_window, cluster_degree_window and cluster_min_rows must be >= 0")
+        self.cluster_window = int(cluster_window)
+        self.cluster_degree_window = int(cluster_degree_window)
+        self.cluster_min_rows = int(cluster_min_rows)
         self._dtype_code = ffi.F64 if np.dtype(layout.dtype) == np.float64 else ffi.F32C
         self._tune_dir = tunecache.default_directory() if tune_cache == "default" else (
             None if tune_cache is None else Path(tune_cache))
@@ -905,7 +920,7 @@ class HipRuntime:
         return ptr_all.to(torch.int32), idx, deg
 
     def _build_artifact(self, ai: int, al, st) -> _HipArtifact:
-        from pygrgl_spmv.backends.hip.host import build_tiles, locality_order, merged_keys
+        from pygrgl_spmv.backends.hip.host import build_tiles, cluster_order, locality_order, merged_keys
 
         torch = _torch()
         lay = self.layout
@@ -929,6 +944,10 @@ class HipRuntime:
         # sharing descendants adjacent (samples stay the prefix of level 0).
         new_pos = locality_order(N, S, off, host_blocks, degree_window=self.degree_window,
                                  degree_min_mean=self.degree_min_mean)
+        if self.cluster_window > 1:
+            new_pos = cluster_order(new_pos, off, host_blocks, window=self.cluster_window,
+                                    degree_window=self.cluster_degree_window, min_mean=self.degree_min_mean,
+                                    min_rows=self.cluster_min_rows)
         new_pos_t = self._to_dev(new_pos)
         up_parts, dn_parts = [], []
         up_stream = [[] for _ in range(L)]
@@ -1120,7 +1139,8 @@ class HipRuntime:
                 "tiles": [bool(self.layout.tiles and self.use_tiles), self.tile_max_staged, self.tile_max_rows,
                           self.tile_max_entries],
                 "panels": [self.use_panels, self.panel_min_mean, self.panel_min_rows, self.panel_rows, self.panel_staged],
-                "order": [self.degree_window, self.degree_min_mean],
+                "order": [self.degree_window, self.degree_min_mean, self.cluster_window, self.cluster_degree_window,
+                          self.cluster_min_rows],
                 "resident": hashlib.blake2b(resident.encode(), digest_size=8).hexdigest()}
 
     def _flush_tune_cache(self) -> None:

@@ -37,6 +37,20 @@ int grg_host_locality_order(int64_t num_nodes, int64_t num_samples, int32_t num_
                             const GrgHostBlock* blocks, int32_t num_blocks, int64_t degree_window, double degree_min_mean,
                             int64_t* new_pos);
 
+/* Min-child order: refines an order new_pos (a permutation inside every level,
+ * as grg_host_locality_order returns) in place, level by level from level 1 up.
+ * Every level above 0 with at least min_rows nodes and a mean child count of at
+ * least min_mean is cut into windows of `window` consecutive positions; inside a
+ * window the nodes are ordered (stably) by the position of their first child in
+ * the refined order, so nodes sharing that child sit side by side and the rows a
+ * workgroup sums together gather more of the same source rows. With
+ * degree_window > 1, aligned runs of degree_window positions inside each window
+ * are then ordered (stably) by child count, which keeps the set of rows of each
+ * run. No reference counterpart. Returns 0 on success. */
+int grg_host_cluster_order(int64_t num_nodes, int32_t num_levels, const int64_t* level_offsets, const GrgHostBlock* blocks,
+                           int32_t num_blocks, int64_t window, int64_t degree_window, double min_mean, int64_t min_rows,
+                           int64_t* new_pos);
+
 /* Tiles of consecutive rows for the tiled launch of libgrg_hip.so (see the
  * tile fields of GrgHipGatherArgs in grg_hip.h), for every level of one sweep
  * configuration at once. No reference counterpart.

@@ -312,6 +312,29 @@ def test_every_kernel_variant(art_a, oracle_a, unroll, rows, es, merged):
         _check(grg.matmul(x, "up", by_individual=True, init="xtx"), O.matmul(oracle_a, x, "up", by_individual=True, init="xtx"))
 
 
+@pytest.mark.parametrize("window,degree_window", [(8, 0), (64, 4)])
+def test_min_child_node_order(art_a, oracle_a, window, degree_window):
+    """Sweeps over the min-child device order (every level above 0 refined): exact for +-1
+    inputs and within 1e-12 of the oracle in every sweep configuration, tuned."""
+    from pygrgl_spmv.backends.hip import HipRuntime
+
+    layout = build_hip_layout([art_a], requirements=full_requirements(max_k_up=8, max_k_down=8))
+    with HipRuntime(layout, cluster_window=window, cluster_degree_window=degree_window, cluster_min_rows=0,
+                    degree_min_mean=0.0, split_min=40, split_chunk=16) as rt:
+        (grg,) = rt.grgs
+        rng = np.random.default_rng(3470 + window)
+        for direction in ("up", "down"):
+            x = rng.choice(np.array([-1.0, 1.0]), size=(5, _cols(oracle_a, direction)))
+            np.testing.assert_array_equal(grg.matmul(x, direction), O.matmul(oracle_a, x, direction))
+        y = rng.standard_normal((8, oracle_a.num_mutations))
+        miss = rng.standard_normal((8, oracle_a.num_mutations))
+        _check(grg.matmul(y, "down", miss=miss), O.matmul(oracle_a, y, "down", miss=miss))
+        x = rng.standard_normal((8, oracle_a.num_individuals))
+        _check(grg.matmul(x, "up", by_individual=True, init="xtx"), O.matmul(oracle_a, x, "up", by_individual=True, init="xtx"))
+        x = rng.standard_normal((3, oracle_a.num_samples))
+        _check(grg.matmul(x, "up", emit_all_nodes=True), O.matmul(oracle_a, x, "up", emit_all_nodes=True))
+
+
 @pytest.mark.parametrize("unroll,rows", [(4, 1), (8, 1), (4, 2)])
 def test_fp32_wide_lanes(art_a, oracle_a, unroll, rows):
     """FP32 values with 8 columns (32 bytes) per lane, k = 16 over 2 lanes."""

@@ -76,3 +76,66 @@ def test_degree_window_rejects_negative(dag):
     N, S, off, blocks, _ = dag
     with pytest.raises(RuntimeError, match="degree_window"):
         locality_order(N, S, off, blocks, degree_window=-1)
+
+
+def _first_child_pos(N, off, blocks, pos):
+    """Position (under ``pos``) of each node's first child, -1 for childless nodes."""
+    mn = np.full(N, np.iinfo(np.int64).max)
+    for d, s, ptr, idx in blocks:
+        rows = np.repeat(np.arange(ptr.size - 1, dtype=np.int64), np.diff(ptr)) + off[d]
+        np.minimum.at(mn, rows, pos[idx.astype(np.int64) + off[s]])
+    return mn
+
+
+@pytest.mark.parametrize("window,degree_window,min_rows", [(8, 0, 0), (64, 0, 0), (64, 8, 0), (32, 4, 40)])
+def test_cluster_order(dag, window, degree_window, min_rows):
+    """Min-child order: each window of a refined level holds the same nodes as before, ordered
+    by the position of their first child (ties in the previous order); aligned degree runs inside
+    it keep their node sets and are ordered by child count; other levels are untouched."""
+    from pygrgl_spmv.backends.hip.host import cluster_order
+
+    N, S, off, blocks, deg = dag
+    base = locality_order(N, S, off, blocks)
+    got = cluster_order(base, off, blocks, window=window, degree_window=degree_window, min_mean=0.0, min_rows=min_rows)
+    _check_permutation(got, N, S, off)
+    assert np.array_equal(cluster_order(base, off, blocks, window=window, degree_window=degree_window, min_mean=0.0,
+                                        min_rows=min_rows), got)  # deterministic
+    node_base = np.empty(N, dtype=np.int64)
+    node_base[base] = np.arange(N)
+    node_got = np.empty(N, dtype=np.int64)
+    node_got[got] = np.arange(N)
+    first = _first_child_pos(N, off, blocks, got)  # children positions are final (lower levels)
+    refined = 0
+    for l in range(off.size - 1):
+        lo, hi = int(off[l]), int(off[l + 1])
+        if l == 0 or hi - lo < max(2, min_rows):
+            assert np.array_equal(node_got[lo:hi], node_base[lo:hi])
+            continue
+        refined += 1
+        for w in range(lo, hi, window):
+            we = min(hi, w + window)
+            assert set(node_got[w:we]) == set(node_base[w:we])
+            # the expected window, restated: stable sort by first-child position, then degree runs
+            exp = sorted(node_base[w:we].tolist(), key=lambda v: first[v])
+            if degree_window > 1:
+                for u in range(0, len(exp), degree_window):
+                    exp[u:u + degree_window] = sorted(exp[u:u + degree_window], key=lambda v: deg[v])
+            assert node_got[w:we].tolist() == exp
+            if degree_window <= 1:
+                assert np.all(np.diff(first[node_got[w:we]]) >= 0)
+    assert refined > 0
+
+
+def test_cluster_order_rejects_bad_arguments(dag):
+    from pygrgl_spmv.backends.hip.host import cluster_order
+
+    N, S, off, blocks, _ = dag
+    base = locality_order(N, S, off, blocks)
+    with pytest.raises(RuntimeError, match="window must be >= 2"):
+        cluster_order(base, off, blocks, window=1)
+    with pytest.raises(RuntimeError, match="degree_window"):
+        cluster_order(base, off, blocks, window=8, degree_window=-1)
+    bad = base.copy()
+    bad[0] = N
+    with pytest.raises(RuntimeError, match="new_pos out of range"):
+        cluster_order(bad, off, blocks, window=8)
