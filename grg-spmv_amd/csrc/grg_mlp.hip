@@ -1,11 +1,14 @@
 // Small fully connected FP32 network kernels (forward, backward, loss, Adam)
 // for the learned CG warm start, gfx950. See include/grg_mlp.h.
 //
-// The network is tiny (tens of inputs, 256-wide hidden layers, batches of a
-// few thousand), so each product is a 16 x 16 output tile per 256-thread
-// workgroup with both operand tiles staged through LDS; every output element
-// is one thread's sum over the inner dimension in ascending order, so results
-// do not depend on the launch (no atomics, no split reductions).
+// The three products of a linear layer (forward, weight gradient, input
+// gradient) run on the matrix cores: each wave owns one 16 x 16 output tile and
+// steps through the inner dimension four at a time with v_mfma_f32_16x16x4f32
+// (FP32 products and sums), four waves per 32 x 32 block of the output. The
+// network is small (tens of inputs, 256-wide hidden layers, batches of a few
+// thousand), so operands are read straight from global memory (they stay in
+// L2). Every output element is one wave's sum in a fixed order, so results do
+// not depend on the launch (no atomics, no split reductions).
 
 #include <hip/hip_runtime.h>
 
@@ -30,61 +33,66 @@ int launched() {
   return 0;
 }
 
-constexpr int kTile = 16;
-
-// Y[b, o] = act(sum_i X[b, i] W[o, i] + bias[o])
-__global__ __launch_bounds__(256) void linear_forward_kernel(const float* __restrict__ X, const float* __restrict__ W,
-                                                             const float* __restrict__ bias, float* __restrict__ Y,
-                                                             int batch, int in, int out, int relu) {
-  __shared__ float xs[kTile][kTile + 1];
-  __shared__ float ws[kTile][kTile + 1];
-  const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
-  const int b = blockIdx.y * kTile + ty;  // row of Y
-  const int o = blockIdx.x * kTile + tx;  // column of Y
-  float acc = 0.f;
-  for (int i0 = 0; i0 < in; i0 += kTile) {
-    const int bx = blockIdx.y * kTile + ty, ix = i0 + tx;
-    xs[ty][tx] = (bx < batch && ix < in) ? X[(int64_t)bx * in + ix] : 0.f;
-    const int ow = blockIdx.x * kTile + ty;
-    ws[ty][tx] = (ow < out && ix < in) ? W[(int64_t)ow * in + ix] : 0.f;
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < kTile; ++t) acc += xs[ty][t] * ws[tx][t];
-    __syncthreads();
-  }
-  if (b < batch && o < out) {
-    float y = acc + (bias ? bias[o] : 0.f);
-    if (relu && y < 0.f) y = 0.f;
-    Y[(int64_t)b * out + o] = y;
-  }
-}
+typedef float MfmaAcc __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float grad_at(const float* __restrict__ Y, const float* __restrict__ dY, int64_t idx, int relu) {
   const float g = dY[idx];
   return (relu && !(Y[idx] > 0.f)) ? 0.f : g;
 }
 
-// dW[o, i] = sum_b G[b, o] X[b, i]
-__global__ __launch_bounds__(256) void linear_dw_kernel(const float* __restrict__ X, const float* __restrict__ Y,
-                                                        const float* __restrict__ dY, float* __restrict__ dW, int batch,
-                                                        int in, int out, int relu) {
-  __shared__ float gs[kTile][kTile + 1];  // [b][o]
-  __shared__ float xs[kTile][kTile + 1];  // [b][i]
-  const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
-  const int o = blockIdx.y * kTile + ty;
-  const int i = blockIdx.x * kTile + tx;
-  float acc = 0.f;
-  for (int b0 = 0; b0 < batch; b0 += kTile) {
-    const int bl = b0 + ty;
-    const int og = blockIdx.y * kTile + tx;
-    gs[ty][tx] = (bl < batch && og < out) ? grad_at(Y, dY, (int64_t)bl * out + og, relu) : 0.f;
-    xs[ty][tx] = (bl < batch && i < in) ? X[(int64_t)bl * in + i] : 0.f;
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < kTile; ++t) acc += gs[t][ty] * xs[t][tx];
-    __syncthreads();
+enum LinearProduct { kForward = 0, kWeightGrad = 1, kInputGrad = 2 };
+
+// C = A B on the matrix cores, with G = dY masked by relu'(Y):
+//   kForward:    Y[b, o]  = act(sum_i X[b, i] W[o, i] + bias[o])  (M = batch, N = out, K = in)
+//   kWeightGrad: dW[o, i] = sum_b G[b, o] X[b, i]                 (M = out, N = in, K = batch)
+//   kInputGrad:  dX[b, i] = sum_o G[b, o] W[o, i]                 (M = batch, N = in, K = out)
+// Wave w of the block owns the 16 x 16 tile at rows m0 = (2 blockIdx.y + w / 2) * 16,
+// columns n0 = (2 blockIdx.x + w % 2) * 16. Per step of four k, lane r + 16 q supplies
+// A[m0 + r][k0 + q] and B[k0 + q][n0 + r] (zero past the matrix edges), and after the
+// last step holds C[m0 + 4 q + t][n0 + r] in accumulator t.
+template <int P>
+__global__ __launch_bounds__(256) void linear_mfma_kernel(const float* __restrict__ X, const float* __restrict__ W,
+                                                          const float* __restrict__ Y, const float* __restrict__ dY,
+                                                          const float* __restrict__ bias, float* __restrict__ C,
+                                                          int batch, int in, int out, int relu) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int M = P == kWeightGrad ? out : batch;
+  const int N = P == kForward ? out : in;
+  const int K = P == kForward ? in : (P == kWeightGrad ? batch : out);
+  const int m0 = ((int)blockIdx.y * 2 + (wave >> 1)) * 16;
+  const int n0 = ((int)blockIdx.x * 2 + (wave & 1)) * 16;
+  const int r = lane & 15, q = lane >> 4;
+  const int m = m0 + r, n = n0 + r;
+  MfmaAcc acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int k = k0 + q;
+    float a = 0.f, bv = 0.f;
+    if (k < K) {
+      if (P == kForward) {
+        if (m < M) a = X[(int64_t)m * in + k];
+        if (n < N) bv = W[(int64_t)n * in + k];
+      } else if (P == kWeightGrad) {
+        if (m < M) a = grad_at(Y, dY, (int64_t)k * out + m, relu);
+        if (n < N) bv = X[(int64_t)k * in + n];
+      } else {
+        if (m < M) a = grad_at(Y, dY, (int64_t)m * out + k, relu);
+        if (n < N) bv = W[(int64_t)k * in + n];
+      }
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc, 0, 0, 0);
   }
-  if (o < out && i < in) dW[(int64_t)o * in + i] = acc;
+  if (n >= N) return;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int mt = m0 + 4 * q + t;
+    if (mt >= M) continue;
+    float y = acc[t];
+    if (P == kForward) {
+      if (bias) y += bias[n];
+      if (relu && y < 0.f) y = 0.f;
+    }
+    C[(int64_t)mt * N + n] = y;
+  }
 }
 
 // dbias[o] = sum_b G[b, o]
@@ -95,29 +103,6 @@ __global__ __launch_bounds__(256) void linear_db_kernel(const float* __restrict_
   float acc = 0.f;
   for (int b = 0; b < batch; ++b) acc += grad_at(Y, dY, (int64_t)b * out + o, relu);
   db[o] = acc;
-}
-
-// dX[b, i] = sum_o G[b, o] W[o, i]
-__global__ __launch_bounds__(256) void linear_dx_kernel(const float* __restrict__ W, const float* __restrict__ Y,
-                                                        const float* __restrict__ dY, float* __restrict__ dX, int batch,
-                                                        int in, int out, int relu) {
-  __shared__ float gs[kTile][kTile + 1];  // [b][o]
-  __shared__ float ws[kTile][kTile + 1];  // [o][i]
-  const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
-  const int b = blockIdx.y * kTile + ty;
-  const int i = blockIdx.x * kTile + tx;
-  float acc = 0.f;
-  for (int o0 = 0; o0 < out; o0 += kTile) {
-    const int og = o0 + tx;
-    gs[ty][tx] = (b < batch && og < out) ? grad_at(Y, dY, (int64_t)b * out + og, relu) : 0.f;
-    const int ow = o0 + ty;
-    ws[ty][tx] = (ow < out && i < in) ? W[(int64_t)ow * in + i] : 0.f;
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < kTile; ++t) acc += gs[ty][t] * ws[t][tx];
-    __syncthreads();
-  }
-  if (b < batch && i < in) dX[(int64_t)b * in + i] = acc;
 }
 
 // One workgroup: dP over all elements and the loss by a fixed-order tree reduction.
@@ -173,9 +158,9 @@ int grg_mlp_linear_forward(const float* X, const float* W, const float* bias, fl
   g_last_error.clear();
   if (!X || !W || !Y) return fail("null pointer");
   if (int rc = check_dims(batch, in, out)) return rc;
-  dim3 grid((out + kTile - 1) / kTile, (batch + kTile - 1) / kTile);
-  hipLaunchKernelGGL(linear_forward_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), X, W, bias, Y, batch,
-                     in, out, relu);
+  dim3 grid((out + 31) / 32, (batch + 31) / 32);
+  hipLaunchKernelGGL(linear_mfma_kernel<kForward>, grid, dim3(256), 0, static_cast<hipStream_t>(stream), X, W,
+                     static_cast<const float*>(nullptr), static_cast<const float*>(nullptr), bias, Y, batch, in, out, relu);
   return launched();
 }
 
@@ -185,14 +170,15 @@ int grg_mlp_linear_backward(const float* X, const float* W, const float* Y, cons
   if (!X || !W || !Y || !dY || !dW || !dbias) return fail("null pointer");
   if (int rc = check_dims(batch, in, out)) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(linear_dw_kernel, dim3((in + kTile - 1) / kTile, (out + kTile - 1) / kTile), dim3(256), 0, s, X, Y,
-                     dY, dW, batch, in, out, relu);
+  hipLaunchKernelGGL(linear_mfma_kernel<kWeightGrad>, dim3((in + 31) / 32, (out + 31) / 32), dim3(256), 0, s, X,
+                     static_cast<const float*>(nullptr), Y, dY, static_cast<const float*>(nullptr), dW, batch, in, out, relu);
   if (int rc = launched()) return rc;
   hipLaunchKernelGGL(linear_db_kernel, dim3((out + 255) / 256), dim3(256), 0, s, Y, dY, dbias, batch, out, relu);
   if (int rc = launched()) return rc;
   if (dX) {
-    hipLaunchKernelGGL(linear_dx_kernel, dim3((in + kTile - 1) / kTile, (batch + kTile - 1) / kTile), dim3(256), 0, s, W,
-                       Y, dY, dX, batch, in, out, relu);
+    hipLaunchKernelGGL(linear_mfma_kernel<kInputGrad>, dim3((in + 31) / 32, (batch + 31) / 32), dim3(256), 0, s,
+                       static_cast<const float*>(nullptr), W, Y, dY, static_cast<const float*>(nullptr), dX, batch, in, out,
+                       relu);
     if (int rc = launched()) return rc;
   }
   return 0;
