@@ -18,37 +18,101 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// Runs f(lo, hi) over [0, n) cut into contiguous ranges on up to 16 host threads (one
+// thread below min_chunk items per thread); an exception in any range is rethrown.
+template <typename F>
+void parallel_ranges(int64_t n, F f, int64_t min_chunk = 1 << 15) {
+  const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+  const int64_t T = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(hw, 16), n / std::max<int64_t>(1, min_chunk)));
+  if (T <= 1) {
+    if (n > 0) f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  std::vector<std::string> errs((size_t)T);
+  for (int64_t t = 0; t < T; ++t) {
+    pool.emplace_back([&, t]() {
+      try {
+        f(n * t / T, n * (t + 1) / T);
+      } catch (const std::exception& e) {
+        errs[(size_t)t] = e.what();
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  for (const auto& e : errs)
+    if (!e.empty()) throw std::invalid_argument(e);
+}
+
+// ids[0, n) ordered by key (ties by id, which is the order the ids come in when they
+// start ascending): contiguous ranges sorted on threads, then merged pairwise.
+void sort_ids_by_key(int64_t* ids, int64_t n, const std::vector<double>& key) {
+  if (n < 2) return;
+  std::vector<std::pair<double, int64_t>> kv((size_t)n);
+  for (int64_t i = 0; i < n; ++i) kv[(size_t)i] = {key[ids[i]], ids[i]};
+  const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+  int64_t T = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(hw, 16), n / (1 << 16)));
+  std::vector<int64_t> cut((size_t)T + 1);
+  for (int64_t t = 0; t <= T; ++t) cut[(size_t)t] = n * t / T;
+  parallel_ranges(T, [&](int64_t t0, int64_t t1) {
+    for (int64_t t = t0; t < t1; ++t) std::sort(kv.begin() + cut[(size_t)t], kv.begin() + cut[(size_t)t + 1]);
+  }, 1);
+  for (int64_t w = 1; w < T; w *= 2) {  // merge runs of w ranges pairwise
+    const int64_t pairs = (T + 2 * w - 1) / (2 * w);
+    parallel_ranges(pairs, [&](int64_t p0, int64_t p1) {
+      for (int64_t p = p0; p < p1; ++p) {
+        const int64_t a = 2 * w * p, m = std::min(T, a + w), e = std::min(T, a + 2 * w);
+        if (m < e)
+          std::inplace_merge(kv.begin() + cut[(size_t)a], kv.begin() + cut[(size_t)m], kv.begin() + cut[(size_t)e]);
+      }
+    }, 1);
+  }
+  for (int64_t i = 0; i < n; ++i) ids[i] = kv[(size_t)i].second;
+}
+
 void locality_order(int64_t N, int64_t S, int32_t L, const int64_t* off, const GrgHostBlock* blocks, int32_t nb,
                     int64_t degree_window, double degree_min_mean, int64_t* new_pos) {
   if (N < 0 || S < 0 || S > N || L < 0) throw std::invalid_argument("bad sizes");
   if (degree_window < 0) throw std::invalid_argument("degree_window < 0");
   if (off[0] != 0 || off[L] != N) throw std::invalid_argument("level offsets do not cover the nodes");
-  // Global children CSR (internal ids), children of a node ordered by source level then column.
-  std::vector<int64_t> deg(N + 1, 0);
+  // Global children CSR (internal ids), children of a node ordered by source level then column:
+  // the blocks of each destination level in source-level order, rows filled on threads.
+  std::vector<std::vector<int32_t>> by_dst((size_t)std::max<int32_t>(L, 0));
   for (int32_t b = 0; b < nb; ++b) {
     const GrgHostBlock& B = blocks[b];
     if (B.dst_level <= B.src_level || B.dst_level >= L) throw std::invalid_argument("block levels out of order");
-    const int64_t rows = off[B.dst_level + 1] - off[B.dst_level];
-    for (int64_t r = 0; r < rows; ++r) deg[off[B.dst_level] + r + 1] += B.indptr[r + 1] - B.indptr[r];
+    by_dst[(size_t)B.dst_level].push_back(b);
+  }
+  for (auto& v : by_dst)
+    std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return blocks[a].src_level < blocks[b].src_level; });
+  std::vector<int64_t> deg(N + 1, 0);
+  for (int32_t d = 0; d < L; ++d) {
+    const int64_t lo = off[d];
+    parallel_ranges(off[d + 1] - lo, [&](int64_t r0, int64_t r1) {
+      for (int32_t bi : by_dst[(size_t)d]) {
+        const GrgHostBlock& B = blocks[bi];
+        for (int64_t r = r0; r < r1; ++r) deg[lo + r + 1] += B.indptr[r + 1] - B.indptr[r];
+      }
+    });
   }
   for (int64_t v = 0; v < N; ++v) deg[v + 1] += deg[v];
   std::vector<int32_t> child((size_t)deg[N]);
-  std::vector<int64_t> fill(deg.begin(), deg.end() - 1);
-  std::vector<int32_t> order(nb);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return blocks[a].src_level < blocks[b].src_level; });
-  for (int32_t bi : order) {
-    const GrgHostBlock& B = blocks[bi];
-    const int64_t rows = off[B.dst_level + 1] - off[B.dst_level];
-    const int64_t src0 = off[B.src_level], src1 = off[B.src_level + 1];
-    for (int64_t r = 0; r < rows; ++r) {
-      int64_t v = off[B.dst_level] + r;
-      for (int64_t j = B.indptr[r]; j < B.indptr[r + 1]; ++j) {
-        int64_t c = src0 + B.indices[j];
-        if (c < src0 || c >= src1) throw std::invalid_argument("block column out of range");
-        child[fill[v]++] = (int32_t)c;
+  for (int32_t d = 0; d < L; ++d) {
+    const int64_t lo = off[d];
+    parallel_ranges(off[d + 1] - lo, [&](int64_t r0, int64_t r1) {
+      for (int64_t r = r0; r < r1; ++r) {
+        int64_t pos = deg[lo + r];
+        for (int32_t bi : by_dst[(size_t)d]) {
+          const GrgHostBlock& B = blocks[bi];
+          const int64_t src0 = off[B.src_level], src1 = off[B.src_level + 1];
+          for (int64_t j = B.indptr[r]; j < B.indptr[r + 1]; ++j) {
+            const int64_t c = src0 + B.indices[j];
+            if (c < src0 || c >= src1) throw std::invalid_argument("block column out of range");
+            child[(size_t)pos++] = (int32_t)c;
+          }
+        }
       }
-    }
+    });
   }
   // Sample keys: first-visit order of a depth-first walk from the top levels down.
   std::vector<double> key(N, -1.0);
@@ -73,17 +137,24 @@ void locality_order(int64_t N, int64_t S, int32_t L, const int64_t* off, const G
       }
     }
   }
-  // Internal keys: mean of children keys, bottom up (children are in lower levels).
+  // Internal keys: mean of children keys, bottom up (children are in lower levels), each
+  // level's nodes on threads.
   const double last = (double)N * 4.0;
   for (int32_t l = 0; l < L; ++l) {
-    for (int64_t v = off[l]; v < off[l + 1]; ++v) {
-      if (v < S) continue;
-      const int64_t n = deg[v + 1] - deg[v];
-      if (n == 0) { key[v] = last + (double)v; continue; }
-      double acc = 0.0;
-      for (int64_t j = deg[v]; j < deg[v + 1]; ++j) acc += key[child[j]];
-      key[v] = acc / (double)n;
-    }
+    const int64_t lo = off[l];
+    parallel_ranges(off[l + 1] - lo, [&](int64_t r0, int64_t r1) {
+      for (int64_t v = lo + r0; v < lo + r1; ++v) {
+        if (v < S) continue;
+        const int64_t n = deg[v + 1] - deg[v];
+        if (n == 0) {
+          key[v] = last + (double)v;
+          continue;
+        }
+        double acc = 0.0;
+        for (int64_t j = deg[v]; j < deg[v + 1]; ++j) acc += key[child[j]];
+        key[v] = acc / (double)n;
+      }
+    });
   }
   std::vector<int64_t> ids;
   for (int32_t l = 0; l < L; ++l) {
@@ -92,21 +163,24 @@ void locality_order(int64_t N, int64_t S, int32_t L, const int64_t* off, const G
     std::iota(ids.begin(), ids.end(), lo);
     if (l == 0) {
       // samples keep the prefix [0, S); other childless nodes follow
-      std::stable_sort(ids.begin(), ids.begin() + (S - lo), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
-      std::stable_sort(ids.begin() + (S - lo), ids.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+      sort_ids_by_key(ids.data(), S - lo, key);
+      sort_ids_by_key(ids.data() + (S - lo), hi - S, key);
     } else {
-      std::stable_sort(ids.begin(), ids.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+      sort_ids_by_key(ids.data(), hi - lo, key);
       // Degree windows: inside each run of degree_window consecutive nodes, order by
       // child count (ties keep key order). A wave sums several consecutive rows in
       // lockstep rounds until its longest row is done, so rows of similar length
       // side by side waste fewer rounds; the window keeps the locality of the keys.
       const int64_t n = hi - lo;
       if (degree_window > 1 && n > 0 && (double)(deg[hi] - deg[lo]) >= degree_min_mean * (double)n) {
-        for (int64_t w = 0; w < n; w += degree_window) {
-          const int64_t we = std::min(n, w + degree_window);
-          std::stable_sort(ids.begin() + w, ids.begin() + we,
-                           [&](int64_t a, int64_t b) { return deg[a + 1] - deg[a] < deg[b + 1] - deg[b]; });
-        }
+        const int64_t nw = (n + degree_window - 1) / degree_window;
+        parallel_ranges(nw, [&](int64_t w0, int64_t w1) {
+          for (int64_t w = w0 * degree_window; w < std::min(n, w1 * degree_window); w += degree_window) {
+            const int64_t we = std::min(n, w + degree_window);
+            std::stable_sort(ids.begin() + w, ids.begin() + we,
+                             [&](int64_t a, int64_t b) { return deg[a + 1] - deg[a] < deg[b + 1] - deg[b]; });
+          }
+        }, 8);
       }
     }
     for (int64_t i = 0; i < hi - lo; ++i) new_pos[ids[i]] = lo + i;
