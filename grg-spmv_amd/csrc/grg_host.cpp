@@ -223,8 +223,12 @@ void cluster_order(int64_t N, int32_t L, const int64_t* off, const GrgHostBlock*
     for (int64_t r = 0; r < n; ++r) e += dg[r];
     if (n < std::max<int64_t>(2, min_rows) || (double)e < min_mean * (double)n) continue;
     // rows (level-local) in their current order
-    ids.resize(n);
-    for (int64_t r = 0; r < n; ++r) ids[new_pos[lo + r] - lo] = r;
+    ids.assign(n, -1);
+    for (int64_t r = 0; r < n; ++r) {
+      const int64_t q = new_pos[lo + r] - lo;
+      if (q < 0 || q >= n || ids[q] >= 0) throw std::invalid_argument("new_pos is not a permutation inside each level");
+      ids[q] = r;
+    }
     for (int64_t w = 0; w < n; w += window) {
       const int64_t we = std::min(n, w + window);
       // rows whose smallest child comes first lead; rows sharing their smallest child end up side by side

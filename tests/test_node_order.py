@@ -139,3 +139,10 @@ def test_cluster_order_rejects_bad_arguments(dag):
     bad[0] = N
     with pytest.raises(RuntimeError, match="new_pos out of range"):
         cluster_order(bad, off, blocks, window=8)
+    # a node moved to another level's positions (a level 2 node given a level 1 position)
+    lo1, lo2 = int(off[1]), int(off[2])
+    swapped = base.copy()
+    a, b = int(np.flatnonzero(base == lo1)[0]), int(np.flatnonzero(base == lo2)[0])
+    swapped[a], swapped[b] = base[b], base[a]
+    with pytest.raises(RuntimeError, match="not a permutation inside each level"):
+        cluster_order(swapped, off, blocks, window=8)
