@@ -42,7 +42,7 @@ from pygrgl_spmv.grg.artifact import ArtifactScan, scan_grg_spmv
 HEAVY_THRESHOLD = 512
 # Rows of gather degree above SPLIT_MIN are cut into chunks of SPLIT_CHUNK entries,
 # one workgroup each, whose FP64 partial rows the level launch then sums.
-SPLIT_MIN = 4096
+SPLIT_MIN = 8192
 SPLIT_CHUNK = 2048
 
 
