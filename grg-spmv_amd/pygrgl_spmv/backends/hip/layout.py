@@ -39,7 +39,7 @@ from pygrgl_spmv.grg.artifact import ArtifactScan, scan_grg_spmv
 
 # Rows of gather degree above this (or above 8x their level's typical batch) get a
 # workgroup of their own.
-HEAVY_THRESHOLD = 512
+HEAVY_THRESHOLD = 64
 # Rows of gather degree above SPLIT_MIN are cut into chunks of SPLIT_CHUNK entries,
 # one workgroup each, whose FP64 partial rows the level launch then sums.
 SPLIT_MIN = 8192
