@@ -189,11 +189,14 @@ class _LevelSchedule:
     split_chunk: int = 0
 
 
+HEAVY_MEAN_FACTOR = 8.0  # rows above this multiple of their level's mean degree are heavy
+
+
 def level_schedule(deg: np.ndarray, lo: int, hi: int, *, heavy_min: int, cursor: int,
                    split_min: int = 0, split_chunk: int = 0):
     """Schedule for rows [lo, hi) of one sweep configuration (``deg`` = gather degree per row).
 
-    Rows above max(heavy_min, 8 x the level's mean degree) get a 256-thread
+    Rows above max(heavy_min, HEAVY_MEAN_FACTOR x the level's mean degree) get a 256-thread
     workgroup of their own, so one long row does not hold a wave of short ones;
     rows above ``split_min`` (when > 0) are further cut into chunks of
     ``split_chunk`` entries. Returns the schedule and its int32 table
@@ -202,7 +205,7 @@ def level_schedule(deg: np.ndarray, lo: int, hi: int, *, heavy_min: int, cursor:
     """
     seg = deg[lo:hi]
     mean = float(seg.mean()) if seg.size else 0.0
-    heavy = int(max(heavy_min, 8.0 * mean))
+    heavy = int(max(heavy_min, HEAVY_MEAN_FACTOR * mean))
     if split_min > 0:
         heavy = min(heavy, int(split_min))  # rows to split are always heavy rows
     heavy_idx = np.flatnonzero(seg > heavy)
