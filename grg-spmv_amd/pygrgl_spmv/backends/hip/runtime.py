@@ -190,6 +190,7 @@ class _LevelSchedule:
 
 
 HEAVY_MEAN_FACTOR = 8.0  # rows above this multiple of their level's mean degree are heavy
+TUNE_INFLIGHT_BYTES = 256  # tuner: gathered bytes in flight per lane (256: 64 registers)
 
 
 def level_schedule(deg: np.ndarray, lo: int, hi: int, *, heavy_min: int, cursor: int,
@@ -1418,7 +1419,7 @@ class HipRuntime:
                             time_shape(cpl, kl, es, 8, 1, 0)
                 _, cpl, kl, es, _, _, _ = min(timings)
                 for unroll, rows in ((4, 1), (16, 1), (4, 2), (8, 2), (4, 4)):
-                    if unroll * rows * cpl * elem > 256:  # keep the loads in flight within 64 registers
+                    if unroll * rows * cpl * elem > TUNE_INFLIGHT_BYTES:  # loads in flight per lane, bounded
                         continue
                     time_shape(cpl, kl, es, unroll, rows, 0)
                 _, cpl, kl, es, unroll, rows, _ = min(timings)
