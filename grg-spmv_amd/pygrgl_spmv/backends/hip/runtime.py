@@ -1138,7 +1138,8 @@ class HipRuntime:
         resident = "".join("1" if b.resident else "0" for b in (*al.blocks_up, *al.blocks_down))
         return {"artifact": tunecache.file_digest(art.path), "dtype": int(self._dtype_code), "abi": ffi.ABI_VERSION,
                 "library": [int(lib.st_size), int(lib.st_mtime_ns)],
-                "device": [str(props.name), str(getattr(props, "gcnArchName", "")), int(props.multi_processor_count)],
+                # the architecture and CU count, not the marketing name (some tools report it empty)
+                "device": [str(getattr(props, "gcnArchName", "")), int(props.multi_processor_count)],
                 "split": [self.split_min, self.split_chunk], "heavy": int(self.layout.heavy_threshold),
                 "tiles": [bool(self.layout.tiles and self.use_tiles), self.tile_max_staged, self.tile_max_rows,
                           self.tile_max_entries],
