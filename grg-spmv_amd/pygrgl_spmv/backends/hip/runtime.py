@@ -1492,7 +1492,9 @@ class HipRuntime:
                     # the row-group form: one wave per 16 consecutive rows, the group's distinct
                     # sources gathered once and summed by MFMA (16-column chunks)
                     n_before = len(timings)
-                    cpls = [c for c, _ in column_shapes(a.dtype, a.k, lds) if c in (1, 2, 4)]
+                    # 1 or 2 columns per lane: the 4-column form ties with them on the 500k x 5M
+                    # sample level but reads 1.7x as much from memory (10.4 against 6.2 GB)
+                    cpls = [c for c, _ in column_shapes(a.dtype, a.k, lds) if c in (1, 2)]
                     for cpl in cpls:
                         for u in (4, 8, 16):
                             time_shape(cpl, ffi.GROUP_ROWS // cpl, 1, u, 1, ffi.FLAG_GROUPS)
