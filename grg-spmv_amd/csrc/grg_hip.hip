@@ -185,6 +185,43 @@ __device__ __forceinline__ void gather_entries(const int32_t* __restrict__ idx, 
   }
 }
 
+// gather_entries for long entry ranges (heavy rows, split-row chunks): the indices
+// of round r + 1 are loaded while the gathers of round r are in flight, so a round
+// costs one memory round trip instead of two. Slots past the end reload the range's
+// first index and gather its row without adding it, so no load depends on a
+// select of a loaded value. Entries are added in the same order as gather_entries.
+template <typename T, int CPL, int U, typename Acc>
+__device__ __forceinline__ void gather_entries_pipelined(const int32_t* __restrict__ idx, const T* __restrict__ src,
+                                                         int64_t ld, int32_t first, int32_t end, int32_t stride,
+                                                         Acc (&acc)[CPL]) {
+  if (first >= end) return;
+  const auto gi = global_ptr(idx);
+  int32_t ii[U];
+#pragma unroll
+  for (int t = 0; t < U; ++t) {
+    const int32_t jt = first + t * stride;
+    ii[t] = gi[jt < end ? jt : first];
+  }
+  for (int32_t j = first; j < end; j += U * stride) {
+    T v[U][CPL];
+#pragma unroll
+    for (int t = 0; t < U; ++t) load_raw_global<T, CPL>(src + (int64_t)ii[t] * ld, v[t]);
+    const int32_t jn = j + U * stride;
+#pragma unroll
+    for (int t = 0; t < U; ++t) {
+      const int32_t jt = jn + t * stride;
+      ii[t] = gi[jt < end ? jt : first];
+    }
+#pragma unroll
+    for (int t = 0; t < U; ++t) {
+      if (j + t * stride < end) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) acc[c].add(v[t][c]);
+      }
+    }
+  }
+}
+
 // R rows at once: the same rounds of U predicated gathers, for R independent
 // entry ranges [j[t], end[t]) side by side, so one round trip serves R short
 // rows. Rounds continue while any of the R ranges has entries left.
@@ -433,8 +470,11 @@ __device__ __forceinline__ void store_row(const GrgHipGatherArgs& a, int64_t row
 // One row by the whole workgroup (heavy rows; rows of direct tiles): 256 / kl
 // edge lanes over the row's segments (or, for a split heavy row h, over its
 // FP64 partial rows), a shuffle + LDS reduction, one store. Uses partial[] and
-// one barrier: callers looping over rows add a barrier between rows.
-template <typename T, int CPL, int U, typename Acc>
+// one barrier: callers looping over rows add a barrier between rows. PIPE: the
+// entry indices of the next round are loaded during the current round's gathers
+// (gather_entries_pipelined); it costs registers, so the kernels whose light
+// work is register-bound (tiles, row groups, streamed) keep the plain rounds.
+template <typename T, int CPL, int U, bool PIPE = false, typename Acc>
 __device__ __forceinline__ void workgroup_row(const GrgHipGatherArgs& a, int64_t row, int64_t h, int by, int kl_shift,
                                               Acc (&partial)[kWavesPerBlock][64]) {
   const int KL = 1 << kl_shift;
@@ -461,7 +501,12 @@ __device__ __forceinline__ void workgroup_row(const GrgHipGatherArgs& a, int64_t
     } else {
       for (int s = 0; s < a.nseg; ++s) {
         const int32_t* ptr = a.seg[s].ptr + (row - a.seg[s].ptr_base);
-        gather_entries<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, ptr[0] + e, ptr[1], ES, acc);
+        if constexpr (PIPE)
+          gather_entries_pipelined<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld,
+                                              ptr[0] + e, ptr[1], ES, acc);
+        else
+          gather_entries<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld, ptr[0] + e, ptr[1],
+                                    ES, acc);
       }
     }
     if (e == 0) add_row_terms<T, CPL>(a, row, col, acc);
@@ -524,8 +569,8 @@ __device__ __forceinline__ void gather_block(const GrgHipGatherArgs& a, int64_t 
         const int64_t b = lo > g ? lo - g : 0;
         const int64_t t = hi - g < n ? hi - g : n;
         if (b < t)
-          gather_entries<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld,
-                                    p0 + (int32_t)b + e, p0 + (int32_t)t, ES, acc);
+          gather_entries_pipelined<T, CPL, U>(a.seg[s].idx, static_cast<const T*>(a.seg[s].src) + col, a.seg[s].ld,
+                                              p0 + (int32_t)b + e, p0 + (int32_t)t, ES, acc);
         g += n;
       }
     }
@@ -663,7 +708,7 @@ __device__ __forceinline__ void gather_block(const GrgHipGatherArgs& a, int64_t 
 
   // ---- heavy path: the whole workgroup on one row ----
   const int64_t h = bx - n_light_blocks;
-  workgroup_row<T, CPL, U>(a, a.heavy_rows[h], h, by, kl_shift, partial);
+  workgroup_row<T, CPL, U, true>(a, a.heavy_rows[h], h, by, kl_shift, partial);
 }
 
 template <typename T, int CPL, int U, int R>
